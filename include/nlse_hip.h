@@ -1,0 +1,184 @@
+/*
+ * nlse_hip.h -- C ABI of the MI355X split-step engine (libnlse_hip.so).
+ *
+ * Drop-in boundary for the backend="GPU" side of Quantum-Optics-LKB/NLSE
+ * v2.3.0.  The reference binds its GPU path in the `backend` property
+ * setter (NLSE/nlse.py:162-186) to three things, each replaced here:
+ *
+ *   VkFFTApp(shape, dtype, ndim, stream, inplace=True, norm=1)
+ *       nlse.py:207-218, executed at nlse.py:435-438
+ *       -> nlse_fft_plan_create / nlse_fft_exec / nlse_fft_plan_destroy
+ *   kernels_gpu.{square_mod, nl_prop, nl_prop_without_V, nl_prop_c,
+ *       nl_prop_without_V_c, rabi_coupling}   NLSE/kernels_gpu.py:4-190
+ *       -> nlse_square_mod, nlse_nl_prop, nlse_nl_prop_c, nlse_rabi_coupling,
+ *          nlse_mul_propagator (the `A *= propagator` of nlse.py:437)
+ *   the split_step / out_field loop            nlse.py:372-619
+ *       -> nlse_plan_* + nlse_run: many fused split steps per call
+ *   the power integrals of the normalisation and of callbacks.norm
+ *       nlse.py:297-303, callbacks.py:54      -> nlse_sumsq (FP64)
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative code on error;
+ *     nothing throws across the boundary.  nlse_last_error() returns the
+ *     message of the last failure on the calling thread.
+ *   - Pointers named `*_dev` are device pointers (hipMalloc'd or torch
+ *     tensors' data_ptr()); everything else is host memory.
+ *   - `stream` is a hipStream_t passed as void*; all work is asynchronous
+ *     on that stream.  Objects are not thread-safe.
+ *   - complex64 = interleaved float (re, im); complex128 = interleaved double.
+ *   - Transform sizes must be powers of two, 8 <= n <= 4096.
+ */
+#ifndef NLSE_HIP_H
+#define NLSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLSE_OK 0
+#define NLSE_ERR_ARG (-1)     /* invalid argument / shape */
+#define NLSE_ERR_HIP (-2)     /* HIP runtime error */
+#define NLSE_ERR_STATE (-3)   /* object not ready (missing tables) */
+
+int nlse_version(void);
+const char* nlse_last_error(void);
+
+/* ------------------------------------------------------------------ FFT
+ * Batched in-place complex64 FFT over the last `ndim` (1..3) axes of a
+ * C-contiguous array of `batch` * prod(dims) elements; dims are given
+ * slowest first.  Forward is unnormalised (kernel e^{-i}); the inverse is
+ * scaled by 1/prod(dims) (VkFFT norm=1, pyFFTW normalise_idft=True).
+ * Replaces pyvkfft.cuda.VkFFTApp, NLSE/nlse.py:207-218,435-438. */
+typedef struct nlse_fft_plan nlse_fft_plan;
+int nlse_fft_plan_create(int ndim, const int64_t* dims, int64_t batch, nlse_fft_plan** out);
+int nlse_fft_exec(nlse_fft_plan* plan, void* a_dev, int inverse, void* stream);
+int nlse_fft_plan_destroy(nlse_fft_plan* plan);
+
+/* ---------------------------------------------------- operator kernels
+ * Elementwise operators with the semantics of NLSE/kernels_gpu.py and
+ * NLSE/kernels_cpu.py.  The field holds n_items * item_size points; every
+ * per-item parameter is a device array of doubles read at index
+ * (item * stride), so stride 0 broadcasts one value (the reference's
+ * (N_real, 1, 1) parameter broadcasting, README.md:147-159).
+ * V_dev (item_size points, shared by all items) may be NULL; v_kind is
+ * 1 for float32 V and 2 for complex64 V. */
+typedef struct nlse_bparam {
+  const double* v_dev;
+  int64_t stride;
+} nlse_bparam;
+
+/* square_mod: A_sq = |A|^2 (kernels_gpu.py:179-190) */
+int nlse_square_mod(const void* A_dev, void* A_sq_dev, int64_t n, void* stream);
+
+/* nl_prop / nl_prop_without_V (kernels_gpu.py:4-65, kernels_cpu.py:5-63):
+ * A *= exp(dz (-alpha [* sat] + i g A_sq sat + i V)), sat = 1/(1+A_sq/Isat).
+ * loss_saturates = 0 gives the CPU semantics (loss not multiplied by sat),
+ * 1 the GPU semantics. */
+int nlse_nl_prop(void* A_dev, const float* A_sq_dev, int64_t n_items, int64_t item_size,
+                 double dz, nlse_bparam alpha, const void* V_dev, int v_kind,
+                 nlse_bparam g, nlse_bparam Isat, int loss_saturates, void* stream);
+
+/* nl_prop_c / nl_prop_without_V_c (kernels_gpu.py:68-139):
+ * A1 *= exp(dz (-alpha sat + i (g11 I1 sat + g12 I2 sat) + i V)),
+ * sat = 1/(1 + I1/Isat1 + I2/Isat2).  A1, I1, I2 may be strided views:
+ * item b, point p is at  base + b*item_stride + p. */
+int nlse_nl_prop_c(void* A1_dev, const float* I1_dev, const float* I2_dev, int64_t n_items,
+                   int64_t item_size, int64_t a_item_stride, int64_t i_item_stride,
+                   double dz, nlse_bparam alpha, const void* V_dev, int v_kind,
+                   nlse_bparam g11, nlse_bparam g12, nlse_bparam Isat1, nlse_bparam Isat2,
+                   void* stream);
+
+/* rabi_coupling (kernels_gpu.py:142-156): A1' = cos(w dz) A1 - i sin(w dz) A2,
+ * A2' = cos(w dz) A2 - i sin(w dz) A1, computed from the old values. */
+int nlse_rabi_coupling(void* A1_dev, void* A2_dev, int64_t n_items, int64_t item_size,
+                       int64_t item_stride, double dz, nlse_bparam omega, void* stream);
+
+/* A *= propagator (nlse.py:437): the propagator (prop_items * item_size
+ * points) is broadcast over the leading items of A; prop_kind 1 =
+ * complex64 (multiply in complex64), 2 = complex128 (multiply in
+ * complex128, result rounded to complex64, nlse_1d.py:104). */
+int nlse_mul_propagator(void* A_dev, const void* prop_dev, int prop_kind, int64_t n_items,
+                        int64_t item_size, int64_t prop_items, void* stream);
+
+/* A[item] *= (float)s[item]: the normalisation A = E_00 * E_in of
+ * _prepare_output_array (nlse.py:304), E_00 rounded to float like numpy's
+ * complex64 arithmetic.  A may be a strided component view (item_stride). */
+int nlse_scale(void* A_dev, int64_t n_items, int64_t item_size, int64_t item_stride, nlse_bparam s,
+               void* stream);
+
+/* Cavity drive of DDGPE.laser_excitation (ddgpe.py:155-156), one term:
+ * A -= F_r * F_t * dz * 1j in complex64 arithmetic, F_r (item_size points)
+ * broadcast over items; A may be a strided component view. */
+int nlse_cavity_drive(void* A_dev, int64_t n_items, int64_t item_size, int64_t item_stride,
+                      const void* Fr_dev, float ft_re, float ft_im, double dz, void* stream);
+
+/* Per-item FP64 sum of |A|^2 over item_size points -> out_dev[n_items]
+ * (power integrals of nlse.py:297-303 and callbacks.py:54). */
+int nlse_sumsq(const void* A_dev, int64_t n_items, int64_t item_size, double* out_dev, void* stream);
+
+/* ------------------------------------------------------ fused engine
+ * A plan runs whole split-step loops (nlse.py:566-588) on one field shape:
+ * 2-D fields (items, [2,] NY, NX).  Its tables are set once; nlse_run
+ * then performs `nsteps` steps from real space to real space, two HBM
+ * sweeps per step. */
+typedef struct nlse_plan nlse_plan;
+
+typedef struct nlse_plan_desc {
+  int ndim;          /* 2 */
+  int64_t ny, nx;    /* transform sizes */
+  int64_t items;     /* leading (broadcast) items */
+  int ncomp;         /* 1, or 2 for CNLSE / DDGPE layouts (..., 2, NY, NX) */
+  int loss_saturates;
+  int row_lines;     /* 0 = default (tuning knob, reserved) */
+  int col_width;     /* 0 = default; columns per block of the column pass */
+} nlse_plan_desc;
+
+/* Single-component nonlinear parameters of one application (one per item):
+ * adz = dz*alpha/2, gdz = dz*k/2*n2*c*eps0, inv_isat = (eps0 c)/(2 I_sat)
+ * (0 for I_sat = inf), vdz = multiplier of the plan's V (dz*k/2 for NLSE,
+ * whose real-space phase is dz*k/2*V, nlse.py:425). */
+typedef struct nlse_nl1 { double adz, gdz, inv_isat, vdz; } nlse_nl1;
+/* Two-component parameters for one component (two per item, component 0
+ * then 1): see nl_prop_c; `vdz` multiplies the shared V. */
+typedef struct nlse_nl2 { double adz, gs_dz, gc_dz, inv_is_s, inv_is_o, vdz; } nlse_nl2;
+
+#define NLSE_SLOT_FULL 0      /* precision="single": N(dz)             */
+#define NLSE_SLOT_HALF_PRE 1  /* precision="double": N before the FFTs */
+#define NLSE_SLOT_HALF_POST 2 /* precision="double": N after the FFTs  */
+
+int nlse_plan_create(const nlse_plan_desc* desc, nlse_plan** out);
+int nlse_plan_destroy(nlse_plan* plan);
+/* Separable propagator: px [ncomp][nx], py [ncomp][ny] complex128 with
+ * P(ky,kx) = complex64(px*py), the product taken in FP64. */
+int nlse_plan_set_propagator_sep(nlse_plan* plan, const double* px, const double* py);
+/* Dense complex64 propagator [ncomp][ny][nx] in device memory owned by the
+ * caller (kept alive by the caller while the plan uses it). */
+int nlse_plan_set_propagator_c64(nlse_plan* plan, const void* prop_dev);
+int nlse_plan_set_nl1(nlse_plan* plan, int slot, const nlse_nl1* params);  /* [items] */
+int nlse_plan_set_nl2(nlse_plan* plan, int slot, const nlse_nl2* params);  /* [items*2] */
+/* Rabi rotation of the single-precision step: cm1 = cos(w dz)-1, sn = sin(w dz)
+ * per item (host arrays), or NULL to disable. */
+int nlse_plan_set_rabi(nlse_plan* plan, const double* cm1, const double* sn);
+/* Potential V as given by the user (device float32 or complex64,
+ * [ny][nx] or [items][ny][nx]); each application scales it by its vdz. */
+int nlse_plan_set_potential(nlse_plan* plan, const void* V_dev, int v_kind, int64_t v_item_stride);
+/* Cavity drive of DDGPE.laser_excitation (ddgpe.py:127-156), applied to
+ * component 1 after every step s: A -= F_r0*F_t0[s]*dz*1j; A -= F_r1*F_t1[s]*dz*1j.
+ * F_r* are device complex64 [ny][nx]; F_t* host complex64 arrays indexed by
+ * the absolute step number; n_t their length.  NULL disables. */
+int nlse_plan_set_drive(nlse_plan* plan, const void* Fr0_dev, const void* Fr1_dev,
+                        const float* Ft0, const float* Ft1, int64_t n_t, double dz);
+/* Run nsteps split steps on A (device, real space in and out).
+ * precision: 0 "single" (Lie), 1 "double" (Strang).  step0 is the absolute
+ * index of the first step (drive time index). */
+int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t step0,
+             void* stream);
+/* Kernel configuration actually used (rows per block, columns per block). */
+int nlse_plan_config(const nlse_plan* plan, int* row_lines, int* col_width);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLSE_HIP_H */

@@ -1,0 +1,132 @@
+"""ctypes binding of libnlse_hip.so (C ABI in include/nlse_hip.h).
+
+The library is the only compute path of this package: if it cannot be
+loaded, every GPU entry point raises ``ImportError`` -- there is no CPU or
+PyTorch fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnlse_hip.so")
+
+c_void_p = C.c_void_p
+c_int = C.c_int
+c_i64 = C.c_int64
+c_double = C.c_double
+
+
+class BParam(C.Structure):
+    """nlse_bparam: device array of doubles read at item*stride."""
+
+    _fields_ = [("v_dev", c_void_p), ("stride", c_i64)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("ndim", c_int),
+        ("ny", c_i64),
+        ("nx", c_i64),
+        ("items", c_i64),
+        ("ncomp", c_int),
+        ("loss_saturates", c_int),
+        ("row_lines", c_int),
+        ("col_width", c_int),
+    ]
+
+
+class NL1(C.Structure):
+    _fields_ = [("adz", c_double), ("gdz", c_double), ("inv_isat", c_double), ("vdz", c_double)]
+
+
+class NL2(C.Structure):
+    _fields_ = [
+        ("adz", c_double),
+        ("gs_dz", c_double),
+        ("gc_dz", c_double),
+        ("inv_is_s", c_double),
+        ("inv_is_o", c_double),
+        ("vdz", c_double),
+    ]
+
+
+# name -> (restype, argtypes); every exported symbol of include/nlse_hip.h
+SIGNATURES = {
+    "nlse_version": (c_int, []),
+    "nlse_last_error": (C.c_char_p, []),
+    "nlse_fft_plan_create": (c_int, [c_int, C.POINTER(c_i64), c_i64, C.POINTER(c_void_p)]),
+    "nlse_fft_exec": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "nlse_fft_plan_destroy": (c_int, [c_void_p]),
+    "nlse_square_mod": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
+    "nlse_nl_prop": (c_int, [c_void_p, c_void_p, c_i64, c_i64, c_double, BParam, c_void_p, c_int,
+                             BParam, BParam, c_int, c_void_p]),
+    "nlse_nl_prop_c": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_i64, c_i64, c_double,
+                               BParam, c_void_p, c_int, BParam, BParam, BParam, BParam, c_void_p]),
+    "nlse_rabi_coupling": (c_int, [c_void_p, c_void_p, c_i64, c_i64, c_i64, c_double, BParam, c_void_p]),
+    "nlse_mul_propagator": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_i64, c_i64, c_void_p]),
+    "nlse_scale": (c_int, [c_void_p, c_i64, c_i64, c_i64, BParam, c_void_p]),
+    "nlse_cavity_drive": (c_int, [c_void_p, c_i64, c_i64, c_i64, c_void_p, C.c_float, C.c_float, c_double,
+                                  c_void_p]),
+    "nlse_sumsq": (c_int, [c_void_p, c_i64, c_i64, c_void_p, c_void_p]),
+    "nlse_plan_create": (c_int, [C.POINTER(PlanDesc), C.POINTER(c_void_p)]),
+    "nlse_plan_destroy": (c_int, [c_void_p]),
+    "nlse_plan_set_propagator_sep": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nlse_plan_set_propagator_c64": (c_int, [c_void_p, c_void_p]),
+    "nlse_plan_set_nl1": (c_int, [c_void_p, c_int, c_void_p]),
+    "nlse_plan_set_nl2": (c_int, [c_void_p, c_int, c_void_p]),
+    "nlse_plan_set_rabi": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nlse_plan_set_potential": (c_int, [c_void_p, c_void_p, c_int, c_i64]),
+    "nlse_plan_set_drive": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_double]),
+    "nlse_run": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
+    "nlse_plan_config": (c_int, [c_void_p, C.POINTER(c_int), C.POINTER(c_int)]),
+}
+
+_lib: Optional[C.CDLL] = None
+_load_error: Optional[str] = None
+
+
+def load() -> C.CDLL:
+    """Load (once) and return the library; raise ImportError if it is absent."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        _load_error = f"{LIB_PATH} not found; build it with `python -m nlse_amd.build`"
+        raise ImportError(_load_error)
+    try:
+        lib = C.CDLL(LIB_PATH)
+    except OSError as e:  # pragma: no cover - depends on the host
+        _load_error = f"cannot load {LIB_PATH}: {e}"
+        raise ImportError(_load_error) from e
+    for name, (res, args) in SIGNATURES.items():
+        try:
+            fn = getattr(lib, name)
+        except AttributeError as e:
+            _load_error = f"{LIB_PATH} lacks symbol {name} (stale build?)"
+            raise ImportError(_load_error) from e
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def available() -> bool:
+    try:
+        load()
+        return True
+    except ImportError:
+        return False
+
+
+class NlseError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = load().nlse_last_error().decode(errors="replace")
+        raise NlseError(f"{what or 'nlse call'} failed ({rc}): {msg}")

@@ -1,0 +1,796 @@
+// libnlse_hip.so: host side of the MI355X split-step engine and its C ABI
+// (declared in include/nlse_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nlse_hip.h"
+#include "passes_2d.hpp"
+
+using namespace nlse;
+
+// ------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(NLSE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+  } while (0)
+
+static inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(NLSE_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return NLSE_OK;
+}
+
+static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// ------------------------------------------------------ twiddle tables
+// One table per (device, line length), layout of Plan<N,P>::tw_offset:
+// stage s (s >= 1), entry [(k-1)*Ns + jm] = exp(-2 pi i jm k / (Ns R)),
+// evaluated in FP64 and stored as {c_hi, s_hi, c_lo, s_lo} with
+// c = c_hi + c_lo to ~2^-48.
+template <int N, int P>
+static std::vector<float4> build_twiddles() {
+  using PL = Plan<N, P>;
+  std::vector<float4> tw(PL::TW_SIZE > 0 ? PL::TW_SIZE : 1);
+  for (int s = 1; s < PL::NSTAGES; ++s) {
+    const int R = PL::radix(s), NS = PL::ns(s), off = PL::tw_offset(s);
+    for (int k = 1; k < R; ++k)
+      for (int jm = 0; jm < NS; ++jm) {
+        // angle = -2 pi * jm*k/(NS*R); reduce the rational first
+        const long num = (long)jm * k, den = (long)NS * R;
+        const double ang = -2.0 * M_PI * (double)num / (double)den;
+        const double c = std::cos(ang), sn = std::sin(ang);
+        const float ch = (float)c, sh = (float)sn;
+        tw[off + (k - 1) * NS + jm] = make_float4(ch, sh, (float)(c - (double)ch), (float)(sn - (double)sh));
+      }
+  }
+  return tw;
+}
+
+static std::mutex g_tw_mu;
+static std::map<std::pair<int, int>, float4*> g_tw;  // (device, N) -> table
+
+template <int N, int P>
+static int get_twiddles(const float4** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, N);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    *out = it->second;
+    return NLSE_OK;
+  }
+  std::vector<float4> h = build_twiddles<N, P>();
+  float4* d = nullptr;
+  HIP_TRY(hipMalloc(&d, h.size() * sizeof(float4)));
+  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
+  g_tw[key] = d;
+  *out = d;
+  return NLSE_OK;
+}
+
+// -------------------------------------------------- kernel dispatchers
+template <int N>
+struct LineCfg {
+  static constexpr int P = N >= 16 ? 16 : N;
+  static constexpr int T = N / P;
+  static constexpr int LINE = Plan<N, P>::LDS_LINE;
+  // rows per block: 256 threads for one component, 128 for pairs
+  static constexpr int LPB1 = (256 / T) >= 1 ? (256 / T) : 1;
+  static constexpr int LPB2 = (128 / T) >= 1 ? (128 / T) : 1;
+  // widest column tile: <= 512 threads (256 VGPRs per lane) and <= 144 KiB of LDS, <= 64
+  static constexpr int WT = 512 / T;
+  static constexpr int WL = (144 * 1024) / (LINE * 8);
+  static constexpr int WM0 = WT < WL ? WT : WL;
+  static constexpr int WMAX = WM0 > 64 ? 64 : WM0;
+};
+
+template <int N>
+static int row_launch(int G, const RowArgs& a, hipStream_t s) {
+  using C = LineCfg<N>;
+  if (G == 1) {
+    constexpr int L = C::LPB1;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    hipLaunchKernelGGL((k_row<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  } else {
+    constexpr int L = C::LPB2;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    hipLaunchKernelGGL((k_row<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  }
+  return launch_status("k_row");
+}
+
+template <int N, int W>
+static void col_launch_w(const ColArgs& a, hipStream_t s) {
+  using C = LineCfg<N>;
+  if constexpr (W <= C::WMAX) {
+    const int64_t blocks = a.nslabs * (a.NX / W);
+    hipLaunchKernelGGL((k_col<N, C::P, W>), dim3((unsigned)blocks), dim3(W * C::T), 0, s, a);
+  }
+}
+
+template <int N>
+static int col_width_for(int64_t ncols, int want) {
+  using C = LineCfg<N>;
+  int w = want > 0 ? want : C::WMAX;
+  if (w > C::WMAX) w = C::WMAX;
+  while (w > 1 && (w > ncols || (ncols % w) != 0)) w >>= 1;
+  return w;
+}
+
+template <int N>
+static int col_launch(int W, const ColArgs& a, hipStream_t s) {
+  switch (W) {
+    case 1: col_launch_w<N, 1>(a, s); break;
+    case 2: col_launch_w<N, 2>(a, s); break;
+    case 4: col_launch_w<N, 4>(a, s); break;
+    case 8: col_launch_w<N, 8>(a, s); break;
+    case 16: col_launch_w<N, 16>(a, s); break;
+    case 32: col_launch_w<N, 32>(a, s); break;
+    case 64: col_launch_w<N, 64>(a, s); break;
+    default: return fail(NLSE_ERR_ARG, "unsupported column width %d", W);
+  }
+  return launch_status("k_col");
+}
+
+// Runtime-N dispatch table.
+struct LineOps {
+  int (*twiddles)(const float4**);
+  int (*row)(int, const RowArgs&, hipStream_t);
+  int (*col)(int, const ColArgs&, hipStream_t);
+  int (*colw)(int64_t, int);
+  int lpb1, lpb2, t;
+};
+
+template <int N>
+static LineOps make_ops() {
+  using C = LineCfg<N>;
+  return LineOps{&get_twiddles<N, C::P>, &row_launch<N>, &col_launch<N>, &col_width_for<N>,
+                 C::LPB1, C::LPB2, C::T};
+}
+
+static const LineOps* line_ops(int64_t n) {
+  static const LineOps ops[] = {make_ops<8>(),   make_ops<16>(),  make_ops<32>(),  make_ops<64>(),
+                                make_ops<128>(), make_ops<256>(), make_ops<512>(), make_ops<1024>(),
+                                make_ops<2048>(), make_ops<4096>()};
+  if (!is_pow2(n) || n < 8 || n > 4096) return nullptr;
+  int l = 0;
+  while ((int64_t(1) << l) < n) ++l;
+  return &ops[l - 3];
+}
+
+static RowArgs row_defaults() {
+  RowArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.out_scale = 1.f;
+  return a;
+}
+
+static ColArgs col_defaults() {
+  ColArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.scale = 1.f;
+  a.out_scale = 1.f;
+  a.ncomp = 1;
+  return a;
+}
+
+// ================================================================= C ABI
+extern "C" {
+
+int nlse_version(void) { return 100; }
+const char* nlse_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- FFT
+struct nlse_fft_plan {
+  int ndim;
+  int64_t dims[3];
+  int64_t batch;
+  const LineOps* ops[3];
+  const float4* tw[3];
+  int colw[3];
+};
+
+int nlse_fft_plan_create(int ndim, const int64_t* dims, int64_t batch, nlse_fft_plan** out) {
+  if (!out) return fail(NLSE_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3 || !dims || batch < 1) return fail(NLSE_ERR_ARG, "bad fft plan arguments");
+  nlse_fft_plan p{};
+  p.ndim = ndim;
+  p.batch = batch;
+  for (int d = 0; d < ndim; ++d) {
+    p.dims[d] = dims[d];
+    p.ops[d] = line_ops(dims[d]);
+    if (!p.ops[d]) return fail(NLSE_ERR_ARG, "fft size %lld unsupported (power of two in [8, 4096])",
+                                (long long)dims[d]);
+    int rc = p.ops[d]->twiddles(&p.tw[d]);
+    if (rc) return rc;
+  }
+  // axis d (not the last) is a strided transform with `cols` contiguous columns
+  for (int d = 0; d + 1 < ndim; ++d) {
+    int64_t cols = 1;
+    for (int e = d + 1; e < ndim; ++e) cols *= dims[e];
+    p.colw[d] = p.ops[d]->colw(cols, 0);
+  }
+  *out = new nlse_fft_plan(p);
+  return NLSE_OK;
+}
+
+int nlse_fft_exec(nlse_fft_plan* p, void* a_dev, int inverse, void* stream) {
+  if (!p || !a_dev) return fail(NLSE_ERR_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t total = 1;
+  for (int d = 0; d < p->ndim; ++d) total *= p->dims[d];
+  const float scale = inverse ? (float)(1.0 / (double)total) : 1.f;
+  const int last = p->ndim - 1;
+  // contiguous axis
+  {
+    RowArgs a = row_defaults();
+    a.A = (float2*)a_dev;
+    a.ngroups = p->batch * (total / p->dims[last]);
+    a.rows_per_item = a.ngroups;
+    a.tw = p->tw[last];
+    a.inverse_first = inverse ? 1 : 0;
+    a.forward_last = inverse ? 0 : 1;
+    a.out_scale = (last == 0) ? scale : 1.f;
+    int rc = p->ops[last]->row(1, a, s);
+    if (rc) return rc;
+  }
+  for (int d = last - 1; d >= 0; --d) {
+    int64_t cols = 1, outer = p->batch;
+    for (int e = d + 1; e < p->ndim; ++e) cols *= p->dims[e];
+    for (int e = 0; e < d; ++e) outer *= p->dims[e];
+    ColArgs a = col_defaults();
+    a.A = (float2*)a_dev;
+    a.nslabs = outer;
+    a.NX = (int)cols;
+    a.tw = p->tw[d];
+    a.do_fwd = inverse ? 0 : 1;
+    a.do_inv = inverse ? 1 : 0;
+    a.out_scale = (d == 0) ? scale : 1.f;
+    int rc = p->ops[d]->col(p->colw[d], a, s);
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
+int nlse_fft_plan_destroy(nlse_fft_plan* p) {
+  delete p;
+  return NLSE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ operator kernels
+namespace {
+
+__device__ __forceinline__ double bp(const nlse_bparam& p, int64_t item) { return p.v_dev[item * p.stride]; }
+
+__global__ void k_square_mod(const float2* __restrict__ A, float* __restrict__ S, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 a = A[i];
+    S[i] = a.x * a.x + a.y * a.y;
+  }
+}
+
+__global__ void k_nl_prop(float2* A, const float* I, int64_t items, int64_t isz, double dz, nlse_bparam alpha,
+                          const void* V, int vkind, nlse_bparam g, nlse_bparam isat, int loss_sat) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz, p = i - item * isz;
+    NLParams1 prm;
+    prm.adz = (float)(dz * bp(alpha, item));
+    prm.gdz = (float)(dz * bp(g, item));
+    const double is = bp(isat, item);
+    prm.inv_isat = __builtin_isinf(is) ? 0.f : (float)(1.0 / is);
+    prm.vdz = (float)dz;
+    float vr, vi;
+    load_v(V, vkind, p, vr, vi);
+    // the standalone operator takes A_sq from the caller (it may hold a
+    // convolved intensity): evaluate with the supplied I
+    const float2 a = A[i];
+    const float Ii = I[i];
+    const float sat = 1.0f / (1.0f + Ii * prm.inv_isat);
+    const float theta = fmaf(prm.gdz * Ii, sat, prm.vdz * vr);
+    const float am = fmaf(prm.vdz, vi, loss_sat ? prm.adz * sat : prm.adz);
+    A[i] = apply_phase_loss(a, am, theta);
+  }
+}
+
+__global__ void k_nl_prop_c(float2* A1, const float* I1, const float* I2, int64_t items, int64_t isz,
+                            int64_t astr, int64_t istr, double dz, nlse_bparam alpha, const void* V, int vkind,
+                            nlse_bparam g11, nlse_bparam g12, nlse_bparam is1, nlse_bparam is2) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz, p = i - item * isz;
+    NLParams2 prm;
+    prm.adz = (float)(dz * bp(alpha, item));
+    prm.gs_dz = (float)(dz * bp(g11, item));
+    prm.gc_dz = (float)(dz * bp(g12, item));
+    const double a1 = bp(is1, item), a2 = bp(is2, item);
+    prm.inv_is_s = __builtin_isinf(a1) ? 0.f : (float)(1.0 / a1);
+    prm.inv_is_o = __builtin_isinf(a2) ? 0.f : (float)(1.0 / a2);
+    prm.vdz = (float)dz;
+    float vr, vi;
+    load_v(V, vkind, p, vr, vi);
+    const int64_t ia = item * astr + p, ii = item * istr + p;
+    A1[ia] = nl2(A1[ia], I1[ii], I2[ii], prm, vr, vi);
+  }
+}
+
+__global__ void k_rabi(float2* A1, float2* A2, int64_t items, int64_t isz, int64_t str, double dz,
+                       nlse_bparam omega) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz, p = i - item * isz;
+    const double w = bp(omega, item) * dz;
+    RabiParams r;
+    const double sh = std::sin(0.5 * w);
+    r.cm1 = (float)(-2.0 * sh * sh);
+    r.sn = (float)std::sin(w);
+    const int64_t k = item * str + p;
+    float2 a1 = A1[k], a2 = A2[k];
+    rabi(a1, a2, r);
+    A1[k] = a1;
+    A2[k] = a2;
+  }
+}
+
+__global__ void k_mul_prop(float2* A, const void* P, int kind, int64_t items, int64_t isz, int64_t pitems) {
+  const int64_t n = items * isz;
+  const int64_t pn = pitems * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pi = i % pn;
+    if (kind == 1)
+      A[i] = prop_c64(A[i], reinterpret_cast<const float2*>(P)[pi], 1.f);
+    else
+      A[i] = prop_c128(A[i], reinterpret_cast<const double2*>(P)[pi], 1.0);
+  }
+}
+
+__global__ void k_scale(float2* A, int64_t items, int64_t isz, int64_t str, nlse_bparam sp) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz, p = i - item * isz;
+    const float s = (float)bp(sp, item);
+    float2 a = A[item * str + p];
+    A[item * str + p] = make_float2(a.x * s, a.y * s);
+  }
+}
+
+__global__ void k_drive(float2* A, int64_t items, int64_t isz, int64_t str, const float2* Fr, float2 ft, float dz) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz, p = i - item * isz;
+    float2 q = cmul(Fr[p], ft);
+    q = make_float2(q.x * dz, q.y * dz);
+    float2 a = A[item * str + p];
+    A[item * str + p] = make_float2(a.x + q.y, a.y - q.x);
+  }
+}
+
+// Per-item FP64 sum of squares: block partials then a per-item pass, in a
+// fixed order (deterministic).
+constexpr int SUMSQ_THREADS = 256;
+constexpr int SUMSQ_CHUNK = SUMSQ_THREADS * 16;
+
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_sumsq_part(const float2* A, int64_t isz, int64_t nchunks,
+                                                             double* part) {
+  const int64_t item = blockIdx.x / nchunks, ch = blockIdx.x % nchunks;
+  const float2* a = A + item * isz;
+  const int64_t beg = ch * SUMSQ_CHUNK, end = beg + SUMSQ_CHUNK < isz ? beg + SUMSQ_CHUNK : isz;
+  double acc = 0.0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += SUMSQ_THREADS) {
+    const float2 v = a[i];
+    acc += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  __shared__ double red[SUMSQ_THREADS];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SUMSQ_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_sumsq_final(const double* part, int64_t nchunks, double* out) {
+  const int64_t item = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < nchunks; i += SUMSQ_THREADS) acc += part[item * nchunks + i];
+  __shared__ double red[SUMSQ_THREADS];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SUMSQ_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[item] = red[0];
+}
+
+// scratch for partial sums, grown on demand, one per device
+std::mutex g_scr_mu;
+std::map<int, std::pair<double*, size_t>> g_scratch;
+
+int scratch_doubles(size_t n, double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  auto& e = g_scratch[dev];
+  if (e.second < n) {
+    if (e.first) HIP_TRY(hipFree(e.first));
+    e.first = nullptr;
+    e.second = 0;
+    HIP_TRY(hipMalloc(&e.first, n * sizeof(double)));
+    e.second = n;
+  }
+  *out = e.first;
+  return NLSE_OK;
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+bool bparam_ok(const nlse_bparam& p) { return p.v_dev != nullptr && p.stride >= 0; }
+
+}  // namespace
+
+extern "C" {
+
+int nlse_square_mod(const void* A, void* S, int64_t n, void* stream) {
+  if (!A || !S || n < 0) return fail(NLSE_ERR_ARG, "bad square_mod arguments");
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_square_mod, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const float2*)A,
+                     (float*)S, n);
+  return launch_status("k_square_mod");
+}
+
+int nlse_nl_prop(void* A, const float* I, int64_t items, int64_t isz, double dz, nlse_bparam alpha,
+                 const void* V, int vkind, nlse_bparam g, nlse_bparam isat, int loss_sat, void* stream) {
+  if (!A || !I || items < 0 || isz < 0 || !bparam_ok(alpha) || !bparam_ok(g) || !bparam_ok(isat))
+    return fail(NLSE_ERR_ARG, "bad nl_prop arguments");
+  if (vkind != V_NONE && vkind != V_REAL && vkind != V_CPLX) return fail(NLSE_ERR_ARG, "bad v_kind %d", vkind);
+  if (vkind != V_NONE && !V) return fail(NLSE_ERR_ARG, "null V with v_kind %d", vkind);
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_nl_prop, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A, I, items, isz,
+                     dz, alpha, V, vkind, g, isat, loss_sat);
+  return launch_status("k_nl_prop");
+}
+
+int nlse_nl_prop_c(void* A1, const float* I1, const float* I2, int64_t items, int64_t isz, int64_t astr,
+                   int64_t istr, double dz, nlse_bparam alpha, const void* V, int vkind, nlse_bparam g11,
+                   nlse_bparam g12, nlse_bparam is1, nlse_bparam is2, void* stream) {
+  if (!A1 || !I1 || !I2 || items < 0 || isz < 0 || !bparam_ok(alpha) || !bparam_ok(g11) || !bparam_ok(g12) ||
+      !bparam_ok(is1) || !bparam_ok(is2))
+    return fail(NLSE_ERR_ARG, "bad nl_prop_c arguments");
+  if (vkind != V_NONE && !V) return fail(NLSE_ERR_ARG, "null V with v_kind %d", vkind);
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_nl_prop_c, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A1, I1, I2, items,
+                     isz, astr, istr, dz, alpha, V, vkind, g11, g12, is1, is2);
+  return launch_status("k_nl_prop_c");
+}
+
+int nlse_rabi_coupling(void* A1, void* A2, int64_t items, int64_t isz, int64_t str, double dz, nlse_bparam omega,
+                       void* stream) {
+  if (!A1 || !A2 || items < 0 || isz < 0 || !bparam_ok(omega)) return fail(NLSE_ERR_ARG, "bad rabi arguments");
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_rabi, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A1, (float2*)A2, items,
+                     isz, str, dz, omega);
+  return launch_status("k_rabi");
+}
+
+int nlse_mul_propagator(void* A, const void* P, int kind, int64_t items, int64_t isz, int64_t pitems,
+                        void* stream) {
+  if (!A || !P || (kind != 1 && kind != 2) || items < 0 || isz < 0 || pitems < 1 || (items % pitems) != 0)
+    return fail(NLSE_ERR_ARG, "bad mul_propagator arguments");
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_mul_prop, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A, P, kind, items,
+                     isz, pitems);
+  return launch_status("k_mul_prop");
+}
+
+int nlse_scale(void* A, int64_t items, int64_t isz, int64_t str, nlse_bparam sp, void* stream) {
+  if (!A || items < 0 || isz < 0 || str < 0 || !bparam_ok(sp)) return fail(NLSE_ERR_ARG, "bad scale arguments");
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A, items, isz, str, sp);
+  return launch_status("k_scale");
+}
+
+int nlse_cavity_drive(void* A, int64_t items, int64_t isz, int64_t str, const void* Fr, float ft_re, float ft_im,
+                      double dz, void* stream) {
+  if (!A || !Fr || items < 0 || isz < 0 || str < 0) return fail(NLSE_ERR_ARG, "bad cavity_drive arguments");
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_drive, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)A, items, isz, str,
+                     (const float2*)Fr, make_float2(ft_re, ft_im), (float)dz);
+  return launch_status("k_drive");
+}
+
+int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* stream) {
+  if (!A || !out || items < 1 || isz < 1) return fail(NLSE_ERR_ARG, "bad sumsq arguments");
+  const int64_t nchunks = (isz + SUMSQ_CHUNK - 1) / SUMSQ_CHUNK;
+  double* part = nullptr;
+  int rc = scratch_doubles((size_t)(items * nchunks), &part);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_sumsq_part, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s, (const float2*)A,
+                     isz, nchunks, part);
+  rc = launch_status("k_sumsq_part");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sumsq_final, dim3((unsigned)items), dim3(SUMSQ_THREADS), 0, s, part, nchunks, out);
+  return launch_status("k_sumsq_final");
+}
+
+// ------------------------------------------------------- fused plan
+struct nlse_plan {
+  nlse_plan_desc d;
+  const LineOps* rops = nullptr;  // rows (NX)
+  const LineOps* cops = nullptr;  // columns (NY)
+  const float4* twx = nullptr;
+  const float4* twy = nullptr;
+  int colw = 1;
+  // propagator
+  int pkind = -1;
+  double2* px = nullptr;
+  double2* py = nullptr;
+  const float2* ptab = nullptr;
+  // nonlinear parameters per slot (device)
+  void* nl[3] = {nullptr, nullptr, nullptr};
+  RabiParams* rabi = nullptr;
+  const void* V = nullptr;
+  int vkind = V_NONE;
+  int64_t v_item_stride = 0;
+  // drive
+  const float2* fr0 = nullptr;
+  const float2* fr1 = nullptr;
+  std::vector<float2> ft0, ft1;
+  float drv_dz = 0.f;
+  bool drive = false;
+};
+
+int nlse_plan_create(const nlse_plan_desc* d, nlse_plan** out) {
+  if (!out || !d) return fail(NLSE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->ndim != 2) return fail(NLSE_ERR_ARG, "nlse_plan supports ndim=2 (got %d)", d->ndim);
+  if (d->items < 1 || (d->ncomp != 1 && d->ncomp != 2))
+    return fail(NLSE_ERR_ARG, "bad items/ncomp (%lld, %d)", (long long)d->items, d->ncomp);
+  nlse_plan* p = new nlse_plan();
+  p->d = *d;
+  p->rops = line_ops(d->nx);
+  p->cops = line_ops(d->ny);
+  if (!p->rops || !p->cops) {
+    delete p;
+    return fail(NLSE_ERR_ARG, "grid %lldx%lld unsupported (powers of two in [8, 4096])", (long long)d->ny,
+                (long long)d->nx);
+  }
+  int rc = p->rops->twiddles(&p->twx);
+  if (!rc) rc = p->cops->twiddles(&p->twy);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  p->colw = p->cops->colw(d->nx, d->col_width);
+  *out = p;
+  return NLSE_OK;
+}
+
+int nlse_plan_destroy(nlse_plan* p) {
+  if (!p) return NLSE_OK;
+  (void)hipFree(p->px);
+  (void)hipFree(p->py);
+  for (void* q : p->nl) (void)hipFree(q);
+  (void)hipFree(p->rabi);
+  delete p;
+  return NLSE_OK;
+}
+
+int nlse_plan_config(const nlse_plan* p, int* row_lines, int* col_width) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  if (row_lines) *row_lines = p->d.ncomp == 1 ? p->rops->lpb1 : p->rops->lpb2;
+  if (col_width) *col_width = p->colw;
+  return NLSE_OK;
+}
+
+static int upload(void** dst, const void* src, size_t bytes) {
+  if (!*dst) HIP_TRY(hipMalloc(dst, bytes));
+  HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return NLSE_OK;
+}
+
+int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* py) {
+  if (!p || !px || !py) return fail(NLSE_ERR_ARG, "null argument");
+  const int nc = p->d.ncomp;
+  int rc = upload((void**)&p->px, px, sizeof(double2) * nc * p->d.nx);
+  if (!rc) rc = upload((void**)&p->py, py, sizeof(double2) * nc * p->d.ny);
+  if (rc) return rc;
+  p->pkind = PROP_SEP64;
+  return NLSE_OK;
+}
+
+int nlse_plan_set_propagator_c64(nlse_plan* p, const void* tab) {
+  if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
+  p->ptab = (const float2*)tab;
+  p->pkind = PROP_C64;
+  return NLSE_OK;
+}
+
+static float inv_or_zero(double v) { return std::isinf(v) ? 0.f : (float)(1.0 / v); }
+
+int nlse_plan_set_nl1(nlse_plan* p, int slot, const nlse_nl1* prm) {
+  if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl1 arguments");
+  if (p->d.ncomp != 1) return fail(NLSE_ERR_ARG, "set_nl1 on a two-component plan");
+  std::vector<NLParams1> h(p->d.items);
+  for (int64_t i = 0; i < p->d.items; ++i) {
+    h[i].adz = (float)prm[i].adz;
+    h[i].gdz = (float)prm[i].gdz;
+    h[i].inv_isat = (float)prm[i].inv_isat;
+    h[i].vdz = (float)prm[i].vdz;
+  }
+  return upload(&p->nl[slot], h.data(), h.size() * sizeof(NLParams1));
+}
+
+int nlse_plan_set_nl2(nlse_plan* p, int slot, const nlse_nl2* prm) {
+  if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl2 arguments");
+  if (p->d.ncomp != 2) return fail(NLSE_ERR_ARG, "set_nl2 on a one-component plan");
+  std::vector<NLParams2> h(p->d.items * 2);
+  for (int64_t i = 0; i < p->d.items * 2; ++i) {
+    h[i].adz = (float)prm[i].adz;
+    h[i].gs_dz = (float)prm[i].gs_dz;
+    h[i].gc_dz = (float)prm[i].gc_dz;
+    h[i].inv_is_s = (float)prm[i].inv_is_s;
+    h[i].inv_is_o = (float)prm[i].inv_is_o;
+    h[i].vdz = (float)prm[i].vdz;
+  }
+  return upload(&p->nl[slot], h.data(), h.size() * sizeof(NLParams2));
+}
+
+int nlse_plan_set_rabi(nlse_plan* p, const double* cm1, const double* sn) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  if (!cm1 || !sn) {
+    (void)hipFree(p->rabi);
+    p->rabi = nullptr;
+    return NLSE_OK;
+  }
+  std::vector<RabiParams> h(p->d.items);
+  for (int64_t i = 0; i < p->d.items; ++i) {
+    h[i].cm1 = (float)cm1[i];
+    h[i].sn = (float)sn[i];
+  }
+  return upload((void**)&p->rabi, h.data(), h.size() * sizeof(RabiParams));
+}
+
+int nlse_plan_set_potential(nlse_plan* p, const void* V, int vkind, int64_t vstride) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  if (vkind != V_NONE && vkind != V_REAL && vkind != V_CPLX) return fail(NLSE_ERR_ARG, "bad v_kind %d", vkind);
+  if (vkind != V_NONE && !V) return fail(NLSE_ERR_ARG, "null V");
+  p->V = V;
+  p->vkind = vkind;
+  p->v_item_stride = vstride;
+  return NLSE_OK;
+}
+
+int nlse_plan_set_drive(nlse_plan* p, const void* fr0, const void* fr1, const float* ft0, const float* ft1,
+                        int64_t n_t, double dz) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  if (!fr0) {
+    p->drive = false;
+    return NLSE_OK;
+  }
+  if (p->d.ncomp != 2 || !fr1 || !ft0 || !ft1 || n_t < 0) return fail(NLSE_ERR_ARG, "bad drive arguments");
+  p->fr0 = (const float2*)fr0;
+  p->fr1 = (const float2*)fr1;
+  p->ft0.assign((const float2*)ft0, (const float2*)ft0 + n_t);
+  p->ft1.assign((const float2*)ft1, (const float2*)ft1 + n_t);
+  p->drv_dz = (float)dz;
+  p->drive = true;
+  return NLSE_OK;
+}
+
+int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, void* stream) {
+  if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
+  if (nsteps == 0) return NLSE_OK;
+  if (p->pkind < 0) return fail(NLSE_ERR_STATE, "propagator not set");
+  const int s_full = NLSE_SLOT_FULL, s_pre = NLSE_SLOT_HALF_PRE, s_post = NLSE_SLOT_HALF_POST;
+  if (precision == 0 && !p->nl[s_full]) return fail(NLSE_ERR_STATE, "nonlinear parameters (single) not set");
+  if (precision == 1 && (!p->nl[s_pre] || !p->nl[s_post]))
+    return fail(NLSE_ERR_STATE, "nonlinear parameters (double) not set");
+  if (p->drive && step0 + nsteps > (int64_t)p->ft0.size())
+    return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
+                (long long)(step0 + nsteps - 1));
+  hipStream_t s = (hipStream_t)stream;
+  const int G = p->d.ncomp;
+  const int64_t ny = p->d.ny, nx = p->d.nx;
+
+  RowArgs r = row_defaults();
+  r.A = (float2*)A;
+  r.rows_per_item = ny;
+  r.ngroups = p->d.items * ny * (G == 2 ? 1 : 1);
+  r.tw = p->twx;
+  r.loss_sat = p->d.loss_saturates;
+  r.V = p->V;
+  r.vkind = p->vkind;
+  r.v_item_stride = p->v_item_stride;
+  r.rabi = p->rabi;
+  r.drv_r0 = p->fr0;
+  r.drv_r1 = p->fr1;
+  r.drv_dz = p->drv_dz;
+
+  ColArgs c = col_defaults();
+  c.A = (float2*)A;
+  c.nslabs = p->d.items * G;
+  c.NX = (int)nx;
+  c.ncomp = G;
+  c.tw = p->twy;
+  c.do_fwd = c.do_prop = c.do_inv = 1;
+  c.pkind = p->pkind;
+  c.px = p->px;
+  c.py = p->py;
+  c.ptab = p->ptab;
+  c.scale = (float)(1.0 / ((double)nx * (double)ny));
+
+  // first row pass: F_x, preceded by the half step of "double"
+  {
+    RowArgs a = r;
+    a.inverse_first = 0;
+    a.forward_last = 1;
+    a.has_b = precision == 1;
+    a.nlp_b = p->nl[s_pre];
+    int rc = p->rops->row(G, a, s);
+    if (rc) return rc;
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    int rc = p->cops->col(p->colw, c, s);
+    if (rc) return rc;
+    const bool last = (k == nsteps - 1);
+    RowArgs a = r;
+    a.inverse_first = 1;
+    a.forward_last = last ? 0 : 1;
+    a.has_a = 1;
+    a.nlp_a = precision == 1 ? p->nl[s_post] : p->nl[s_full];
+    a.has_rabi = (precision == 0 && p->rabi != nullptr && G == 2) ? 1 : 0;
+    a.has_drive = p->drive ? 1 : 0;
+    if (p->drive) {
+      a.drv_t0 = p->ft0[step0 + k];
+      a.drv_t1 = p->ft1[step0 + k];
+    }
+    a.has_b = (precision == 1 && !last) ? 1 : 0;
+    a.nlp_b = p->nl[s_pre];
+    rc = p->rops->row(G, a, s);
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+// Line FFT building blocks for gfx950 (CDNA4): complex64 Stockham transform
+// of a power-of-two line held as a "natural set" in registers.
+//
+// A line of N points is owned by T = N/P threads; thread t holds
+//     v[m] = x[t + m*T],   m = 0 .. P-1.
+// Stage s (radix R, Ns = product of the radices of the stages before it)
+// evaluates N/R butterflies j = t + b*T (b < P/R), whose inputs are exactly
+// v[b + k*P/R] (k < R) -- a thread's own registers -- and whose outputs go to
+// the Stockham positions (j/Ns)*Ns*R + j%Ns + k*Ns.  The first stage reads
+// registers loaded straight from HBM, the last stage (Ns = N/R) lands again
+// on the natural set, so a forward transform, a pointwise operator and an
+// inverse transform chain without an extra exchange.  Middle stages exchange
+// through LDS with one float2 of padding every 16 entries, which keeps the
+// b64 accesses of each 16-lane group on distinct banks.
+//
+// Accuracy: the long split-step runs accumulate any *systematic* error of
+// the transform (a gain bias of 1e-7 per forward/inverse pair becomes 1e-4
+// after 1000 steps).  The internal radix-8/16 constants (1/sqrt2, cos pi/8,
+// sin pi/8) are therefore applied as hi+lo float pairs with FMAs, so the
+// only error left is round-to-nearest of data-dependent values.  The stage
+// twiddles are stored as hi+lo float pairs too (exact to ~2^-48) and applied
+// with a compensated complex multiply: a table of twiddles rounded to
+// complex64 has a mean |w|^2 - 1 of about -1e-8 per stage (its few short
+// stages are dominated by near-axis values whose cos rounds down), which a
+// forward/inverse pair turns into a -5e-8 gain per split step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nlse {
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+// a * (-i)
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// Split constants: value = HI + LO to ~2^-48 relative.
+#define NLSE_SPLIT(name, val)                                  \
+  constexpr float name##_HI = (float)(val);                    \
+  constexpr float name##_LO = (float)((val) - (double)(float)(val));
+NLSE_SPLIT(R2, 0.70710678118654752440084436210484904)   // 1/sqrt(2)
+NLSE_SPLIT(C8, 0.92387953251128675612818318939678829)   // cos(pi/8)
+NLSE_SPLIT(S8, 0.38268343236508977172845998403039887)   // sin(pi/8)
+#undef NLSE_SPLIT
+
+// x * c for a split constant, one rounding of a data-dependent value.
+#define NLSE_CMUL_S(x, name) fmaf((x), name##_HI, (x) * name##_LO)
+
+// a * (c - i s) with split c, s: (a.x c + a.y s) + i (a.y c - a.x s)
+#define NLSE_ROT(a, C, S)                                                          \
+  make_float2(fmaf((a).x, C##_HI, fmaf((a).y, S##_HI, fmaf((a).x, C##_LO, (a).y * S##_LO))), \
+              fmaf((a).y, C##_HI, fmaf(-(a).x, S##_HI, fmaf((a).y, C##_LO, -(a).x * S##_LO))))
+
+// a * w for a stage twiddle w = (c_hi + c_lo) + i (s_hi + s_lo),
+// tw = {c_hi, s_hi, c_lo, s_lo}
+__device__ __forceinline__ float2 cmul_tw(float2 a, float4 w) {
+  return make_float2(fmaf(a.x, w.x, fmaf(-a.y, w.y, fmaf(a.x, w.z, -a.y * w.w))),
+                     fmaf(a.x, w.y, fmaf(a.y, w.x, fmaf(a.x, w.w, a.y * w.z))));
+}
+
+// a * W8^1 = a * (1 - i)/sqrt2
+__device__ __forceinline__ float2 mul_w8_1(float2 a) {
+  return make_float2(NLSE_CMUL_S(a.x + a.y, R2), NLSE_CMUL_S(a.y - a.x, R2));
+}
+// a * W8^3 = a * (-1 - i)/sqrt2
+__device__ __forceinline__ float2 mul_w8_3(float2 a) {
+  return make_float2(NLSE_CMUL_S(a.y - a.x, R2), NLSE_CMUL_S(-(a.x + a.y), R2));
+}
+
+// ---------------------------------------------------------- butterflies
+// Forward DFTs (kernel exp(-2 pi i n k / R)), in place on x[0..R-1].
+__device__ __forceinline__ void dft2(float2* x) {
+  float2 a = x[0], b = x[1];
+  x[0] = cadd(a, b);
+  x[1] = csub(a, b);
+}
+
+__device__ __forceinline__ void dft4(float2& a, float2& b, float2& c, float2& d) {
+  float2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = mul_mi(csub(b, d));
+  a = cadd(t0, t2);  // X0
+  c = csub(t0, t2);  // X2
+  b = cadd(t1, t3);  // X1
+  d = csub(t1, t3);  // X3
+}
+
+__device__ __forceinline__ void dft4(float2* x) { dft4(x[0], x[1], x[2], x[3]); }
+
+__device__ __forceinline__ void dft8(float2* x) {
+  // n = 2 n1 + n2, k = k1 + 4 k2
+  float2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  float2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  o1 = mul_w8_1(o1);
+  o2 = mul_mi(o2);
+  o3 = mul_w8_3(o3);
+  x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
+  x[1] = cadd(e1, o1); x[5] = csub(e1, o1);
+  x[2] = cadd(e2, o2); x[6] = csub(e2, o2);
+  x[3] = cadd(e3, o3); x[7] = csub(e3, o3);
+}
+
+__device__ __forceinline__ void dft16(float2* x) {
+  // n = 4 n1 + n2, k = k1 + 4 k2: radix-4 over n1, twiddle W16^(n2 k1),
+  // radix-4 over n2.
+  float2 y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 a = x[n2], b = x[4 + n2], c = x[8 + n2], d = x[12 + n2];
+    dft4(a, b, c, d);
+    y[n2 * 4 + 0] = a; y[n2 * 4 + 1] = b; y[n2 * 4 + 2] = c; y[n2 * 4 + 3] = d;
+  }
+  // W16^1 = c8 - i s8, W16^2 = W8^1, W16^3 = s8 - i c8,
+  // W16^4 = -i, W16^6 = W8^3, W16^9 = -c8 + i s8 = -(W16^1)
+  y[1 * 4 + 1] = NLSE_ROT(y[5], C8, S8);
+  y[1 * 4 + 2] = mul_w8_1(y[6]);
+  y[1 * 4 + 3] = NLSE_ROT(y[7], S8, C8);
+  y[2 * 4 + 1] = mul_w8_1(y[9]);
+  y[2 * 4 + 2] = mul_mi(y[10]);
+  y[2 * 4 + 3] = mul_w8_3(y[11]);
+  y[3 * 4 + 1] = NLSE_ROT(y[13], S8, C8);
+  y[3 * 4 + 2] = mul_w8_3(y[14]);
+  {
+    float2 r = NLSE_ROT(y[15], C8, S8);
+    y[15] = make_float2(-r.x, -r.y);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 a = y[0 * 4 + k1], b = y[1 * 4 + k1], c = y[2 * 4 + k1], d = y[3 * 4 + k1];
+    dft4(a, b, c, d);
+    x[k1] = a; x[k1 + 4] = b; x[k1 + 8] = c; x[k1 + 12] = d;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2* x) {
+  if constexpr (R == 2) dft2(x);
+  else if constexpr (R == 4) dft4(x);
+  else if constexpr (R == 8) dft8(x);
+  else if constexpr (R == 16) dft16(x);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ------------------------------------------------------------ radix plan
+// Stages for N = 2^n with at most P points per thread: the remainder radix
+// (2, 4 or 8) first, then radix-P stages.  P is 4, 8 or 16.
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+
+template <int N, int P>
+struct Plan {
+  static constexpr int LOGN = ilog2c(N);
+  static constexpr int LOGP = ilog2c(P);
+  static_assert((1 << LOGN) == N, "N must be a power of two");
+  static_assert(P == 4 || P == 8 || P == 16, "P must be 4, 8 or 16");
+  static_assert(N >= P, "line shorter than the per-thread set");
+  static constexpr int FULL = LOGN / LOGP;        // radix-P stages
+  static constexpr int REM = LOGN % LOGP;         // log2 of the remainder radix
+  static constexpr int NSTAGES = FULL + (REM ? 1 : 0);
+  static constexpr int T = N / P;                 // threads per line
+  __host__ __device__ static constexpr int radix(int s) {
+    return (REM && s == 0) ? (1 << REM) : P;
+  }
+  __host__ __device__ static constexpr int ns(int s) {  // product of radices before s
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= radix(i);
+    return p;
+  }
+  // Twiddle table offset of stage s (stages with Ns > 1), layout [k-1][jm].
+  __host__ __device__ static constexpr int tw_offset(int s) {
+    int off = 0;
+    for (int i = 1; i < s; ++i) off += (radix(i) - 1) * ns(i);
+    return off;
+  }
+  static constexpr int TW_SIZE = tw_offset(NSTAGES);
+  // padded LDS line length
+  static constexpr int LDS_LINE = N + N / 16 + (N < 16 ? 1 : 0);
+};
+
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+
+// One Stockham stage on G lines owned by the same thread.
+//   FROM_LDS: inputs come from LDS (else from registers v)
+//   TO_LDS:   outputs go to LDS (else, last stage, back to v)
+template <int N, int P, int S, int G>
+__device__ __forceinline__ void fft_stage(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
+                                          float2* lds, int lds_stride) {
+  using PL = Plan<N, P>;
+  constexpr int R = PL::radix(S);
+  constexpr int NS = PL::ns(S);
+  constexpr int NB = P / R;  // butterflies per thread
+  constexpr bool LAST = (S == PL::NSTAGES - 1);
+  constexpr int T = PL::T;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    const int jm = j & (NS - 1);
+    float4 tws[R > 1 ? R - 1 : 1];
+    if constexpr (NS > 1) {
+      const float4* twst = tw + PL::tw_offset(S);
+#pragma unroll
+      for (int k = 1; k < R; ++k) tws[k - 1] = twst[(k - 1) * NS + jm];
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      float2 x[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] = v[g][b + k * NB];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int k = 1; k < R; ++k) x[k] = cmul_tw(x[k], tws[k - 1]);
+      }
+      dft<R>(x);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[g][b + k * NB] = x[k];
+      } else {
+        const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+        for (int k = 0; k < R; ++k) lds[g * lds_stride + lds_pad(base + k * NS)] = x[k];
+      }
+    }
+  }
+}
+
+template <int N, int P, int G>
+__device__ __forceinline__ void lds_gather(float2 (&v)[G][P], int t, const float2* lds, int lds_stride) {
+  constexpr int T = Plan<N, P>::T;
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[g][m] = lds[g * lds_stride + lds_pad(t + m * T)];
+}
+
+// Barrier between LDS phases.  WG-wide: the lines of a block may span waves.
+__device__ __forceinline__ void lds_sync() { __syncthreads(); }
+
+template <int N, int P, int S, int G>
+__device__ __forceinline__ void fft_stages_from(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
+                                                float2* lds, int lds_stride) {
+  using PL = Plan<N, P>;
+  if constexpr (S < PL::NSTAGES) {
+    fft_stage<N, P, S, G>(v, t, tw, lds, lds_stride);
+    if constexpr (S < PL::NSTAGES - 1) {
+      lds_sync();
+      lds_gather<N, P, G>(v, t, lds, lds_stride);
+      lds_sync();
+    }
+    fft_stages_from<N, P, S + 1, G>(v, t, tw, lds, lds_stride);
+  }
+}
+
+// Forward (INV=false) or unnormalised inverse (INV=true) DFT of G lines,
+// natural set in, natural set out.  The inverse is conj(F(conj(x))).
+template <int N, int P, int G, bool INV>
+__device__ __forceinline__ void fft_line(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
+                                         float2* lds, int lds_stride) {
+  if constexpr (INV) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m].y = -v[g][m].y;
+  }
+  fft_stages_from<N, P, 0, G>(v, t, tw, lds, lds_stride);
+  if constexpr (INV) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m].y = -v[g][m].y;
+  }
+}
+
+}  // namespace nlse

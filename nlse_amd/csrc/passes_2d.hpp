@@ -1,0 +1,196 @@
+// Fused line passes of the 2-D split step.
+//
+// Field layout in HBM: complex64, row-major (..., [2,] NY, NX), leading dims
+// flattened into "items"; a 2-component field stores component c of item b
+// at slab b*2 + c.  Between passes the field lives in the mixed domain
+// B = F_x(A) (x spectral, y real).  One split step is then two sweeps:
+//
+//   column pass  C = I_y * diag(P/(NX NY)) * F_y      (strided columns)
+//   row pass     R = F_x * Nl * I_x                    (contiguous rows)
+//
+// where Nl is the real-space operator of the step (nl_prop[_c], Rabi,
+// drive).  The first row pass of a run omits I_x, the last omits F_x.
+// Every sweep reads and writes the field once: 16 B per point, plus the
+// real-space side inputs in the row pass.
+#pragma once
+#include "fft_line.hpp"
+#include "step_ops.hpp"
+
+namespace nlse {
+
+enum : int { PROP_SEP64 = 0, PROP_C64 = 1 };
+
+struct RowArgs {
+  float2* A;
+  int64_t ngroups;        // rows (G=1) or row pairs (G=2)
+  int64_t rows_per_item;  // NY: rows per item and component
+  const float4* tw;       // twiddles of the row length (hi+lo records)
+  int inverse_first;      // apply I_x (unnormalised) first
+  int forward_last;       // apply F_x last
+  int has_a;              // first nonlinear application (end of step k)
+  int has_rabi;           // Rabi rotation after application a (G=2)
+  int has_drive;          // cavity drive after the step (G=2, DDGPE)
+  int has_b;              // second nonlinear application (start of step k+1)
+  int loss_sat;           // single-component loss multiplied by sat
+  const void* nlp_a;      // NLParams1[items] or NLParams2[items*2]
+  const void* nlp_b;
+  const RabiParams* rabi; // [items]
+  const void* V;          // potential V' (k/2 V), broadcast over items
+  int vkind;
+  int64_t v_item_stride;  // elements between items of V (0: shared)
+  // drive: A[...,1,:,:] -= F_r * F_t[step] * dz * 1j for (pump, probe)
+  const float2* drv_r0;
+  const float2* drv_r1;
+  float2 drv_t0;
+  float2 drv_t1;
+  float drv_dz;
+  float out_scale;        // applied to the stored result (1 or 1/N)
+};
+
+template <int N, int P, int LPB, int G>
+__global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = PL::LDS_LINE;
+  __shared__ float2 lds[LPB * G * LINE];
+
+  const int li = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t gi = (int64_t)blockIdx.x * LPB + li;
+  const bool active = gi < a.ngroups;
+  const int64_t gsafe = active ? gi : 0;
+  const int64_t item = gsafe / a.rows_per_item;
+  const int64_t y = gsafe - item * a.rows_per_item;
+  int64_t off[G];
+  if constexpr (G == 1) {
+    off[0] = gsafe * N;
+  } else {
+    off[0] = ((item * 2) * a.rows_per_item + y) * N;
+    off[1] = off[0] + a.rows_per_item * N;
+  }
+  float2* lline = lds + li * G * LINE;
+
+  float2 v[G][P];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[off[g] + t + m * T] : make_float2(0.f, 0.f);
+
+  if (a.inverse_first) fft_line<N, P, G, true>(v, t, a.tw, lline, LINE);
+
+  // real-space operator on the natural set x = t + m*T
+  if (a.has_a || a.has_b || a.has_drive) {
+    const int64_t vbase = item * a.v_item_stride + y * N;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int x = t + m * T;
+      float vr, vi;
+      load_v(a.V, a.vkind, vbase + x, vr, vi);
+      if constexpr (G == 1) {
+        if (a.has_a) v[0][m] = nl1(v[0][m], reinterpret_cast<const NLParams1*>(a.nlp_a)[item], a.loss_sat, vr, vi);
+        if (a.has_b) v[0][m] = nl1(v[0][m], reinterpret_cast<const NLParams1*>(a.nlp_b)[item], a.loss_sat, vr, vi);
+      } else {
+        if (a.has_a) {
+          const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_a) + 2 * item;
+          const float I1 = v[0][m].x * v[0][m].x + v[0][m].y * v[0][m].y;
+          const float I2 = v[1][m].x * v[1][m].x + v[1][m].y * v[1][m].y;
+          v[0][m] = nl2(v[0][m], I1, I2, p[0], vr, vi);
+          v[1][m] = nl2(v[1][m], I2, I1, p[1], vr, vi);
+          if (a.has_rabi) rabi(v[0][m], v[1][m], a.rabi[item]);
+        }
+        if (a.has_drive) {
+          const int64_t ri = y * N + x;
+          float2 q = cmul(a.drv_r0[ri], a.drv_t0);
+          q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+          v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+          q = cmul(a.drv_r1[ri], a.drv_t1);
+          q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+          v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+        }
+        if (a.has_b) {
+          const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_b) + 2 * item;
+          const float I1 = v[0][m].x * v[0][m].x + v[0][m].y * v[0][m].y;
+          const float I2 = v[1][m].x * v[1][m].x + v[1][m].y * v[1][m].y;
+          v[0][m] = nl2(v[0][m], I1, I2, p[0], vr, vi);
+          v[1][m] = nl2(v[1][m], I2, I1, p[1], vr, vi);
+        }
+      }
+    }
+  }
+
+  if (a.forward_last) fft_line<N, P, G, false>(v, t, a.tw, lline, LINE);
+
+  if (active) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        float2 r = v[g][m];
+        r.x *= a.out_scale;
+        r.y *= a.out_scale;
+        a.A[off[g] + t + m * T] = r;
+      }
+  }
+}
+
+struct ColArgs {
+  float2* A;
+  int64_t nslabs;      // items * components
+  int NX;              // columns per slab
+  int ncomp;           // propagator component = slab % ncomp
+  const float4* tw;    // twiddles of the column length (hi+lo records)
+  int do_fwd;          // F_y
+  int do_prop;         // multiply by the propagator (with `scale`)
+  int do_inv;          // I_y
+  int pkind;           // PROP_SEP64 or PROP_C64
+  const double2* px;   // [ncomp][NX]   separable factors exp(-i phi_x)
+  const double2* py;   // [ncomp][NY]   exp(-i phi_y)
+  const float2* ptab;  // [ncomp][NY][NX] dense complex64 propagator
+  float scale;         // 1/(NX NY) folded into the propagator
+  float out_scale;     // applied to the stored result when no propagator
+};
+
+template <int N, int P, int W>
+__global__ __launch_bounds__(W*(N / P)) void k_col(ColArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = PL::LDS_LINE;
+  __shared__ float2 lds[W * LINE];
+
+  const int cw = threadIdx.x % W;
+  const int t = threadIdx.x / W;
+  const int tiles = a.NX / W;
+  const int64_t slab = blockIdx.x / tiles;
+  const int c = (blockIdx.x % tiles) * W + cw;
+  const int64_t base = slab * (int64_t)N * a.NX + c;
+  float2* lline = lds + cw * LINE;
+
+  float2 v[1][P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[0][m] = a.A[base + (int64_t)(t + m * T) * a.NX];
+
+  if (a.do_fwd) fft_line<N, P, 1, false>(v, t, a.tw, lline, LINE);
+  if (a.do_prop) {
+    const int comp = (int)(slab % a.ncomp);
+    if (a.pkind == PROP_SEP64) {
+      const double2 px = a.px[comp * a.NX + c];
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[0][m] = prop_sep_c64(v[0][m], px, a.py[comp * N + t + m * T], a.scale);
+    } else {
+      const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c;
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[0][m] = prop_c64(v[0][m], pt[(int64_t)(t + m * T) * a.NX], a.scale);
+    }
+  }
+  if (a.do_inv) fft_line<N, P, 1, true>(v, t, a.tw, lline, LINE);
+
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    float2 r = v[0][m];
+    r.x *= a.out_scale;
+    r.y *= a.out_scale;
+    a.A[base + (int64_t)(t + m * T) * a.NX] = r;
+  }
+}
+
+}  // namespace nlse

@@ -1,0 +1,143 @@
+"""Python handle of the fused split-step plan (nlse_plan_* in nlse_hip.h)."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+
+SLOT_FULL, SLOT_HALF_PRE, SLOT_HALF_POST = 0, 1, 2
+
+NL1_DTYPE = np.dtype([("adz", "f8"), ("gdz", "f8"), ("inv_isat", "f8"), ("vdz", "f8")])
+NL2_DTYPE = np.dtype([("adz", "f8"), ("gs_dz", "f8"), ("gc_dz", "f8"), ("inv_is_s", "f8"),
+                      ("inv_is_o", "f8"), ("vdz", "f8")])
+
+
+def inv_or_zero(x):
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        return np.where(np.isinf(x), 0.0, 1.0 / x)
+
+
+class Plan2D:
+    """Fused 2-D engine for fields (items, [2,] ny, nx) of complex64."""
+
+    def __init__(self, ny: int, nx: int, items: int, ncomp: int = 1, loss_saturates: bool = False,
+                 col_width: int = 0):
+        self.ny, self.nx, self.items, self.ncomp = int(ny), int(nx), int(items), int(ncomp)
+        desc = _lib.PlanDesc(2, self.ny, self.nx, self.items, self.ncomp, int(bool(loss_saturates)), 0,
+                             int(col_width))
+        h = C.c_void_p()
+        _lib.check(_lib.load().nlse_plan_create(C.byref(desc), C.byref(h)), "plan_create")
+        self._h = h
+        self._keep = {}
+        self._last = {}
+
+    # -- tables ----------------------------------------------------------
+    def set_propagator_sep(self, px: np.ndarray, py: np.ndarray) -> None:
+        px = np.ascontiguousarray(np.asarray(px, dtype=np.complex128).reshape(self.ncomp, self.nx))
+        py = np.ascontiguousarray(np.asarray(py, dtype=np.complex128).reshape(self.ncomp, self.ny))
+        key = ("sep", px.tobytes(), py.tobytes())
+        if self._last.get("prop") == key:
+            return
+        _lib.check(_lib.load().nlse_plan_set_propagator_sep(self._h, px.ctypes.data, py.ctypes.data), "set_prop")
+        self._last["prop"] = key
+
+    def set_propagator_c64(self, table: torch.Tensor) -> None:
+        t = table.to(torch.complex64).contiguous()
+        if t.numel() != self.ncomp * self.ny * self.nx:
+            raise ValueError("dense propagator must have ncomp*ny*nx points")
+        self._keep["prop"] = t
+        _lib.check(_lib.load().nlse_plan_set_propagator_c64(self._h, t.data_ptr()), "set_prop_c64")
+        self._last["prop"] = ("c64", t.data_ptr())
+
+    def _set_struct(self, fn, slot, arr, dtype, n):
+        arr = np.ascontiguousarray(arr, dtype=dtype)
+        if arr.shape != (n,):
+            raise ValueError(f"expected {n} parameter records, got {arr.shape}")
+        key = ("nl", slot)
+        b = arr.tobytes()
+        if self._last.get(key) == b:
+            return
+        _lib.check(fn(self._h, slot, arr.ctypes.data), "set_nl")
+        self._last[key] = b
+
+    def set_nl1(self, slot: int, params: np.ndarray) -> None:
+        self._set_struct(_lib.load().nlse_plan_set_nl1, slot, params, NL1_DTYPE, self.items)
+
+    def set_nl2(self, slot: int, params: np.ndarray) -> None:
+        self._set_struct(_lib.load().nlse_plan_set_nl2, slot, params, NL2_DTYPE, 2 * self.items)
+
+    def set_rabi(self, cm1, sn) -> None:
+        lib = _lib.load()
+        if cm1 is None:
+            _lib.check(lib.nlse_plan_set_rabi(self._h, None, None), "set_rabi")
+            self._last.pop("rabi", None)
+            return
+        cm1 = np.ascontiguousarray(np.broadcast_to(np.asarray(cm1, np.float64), (self.items,)))
+        sn = np.ascontiguousarray(np.broadcast_to(np.asarray(sn, np.float64), (self.items,)))
+        key = cm1.tobytes() + sn.tobytes()
+        if self._last.get("rabi") == key:
+            return
+        _lib.check(lib.nlse_plan_set_rabi(self._h, cm1.ctypes.data, sn.ctypes.data), "set_rabi")
+        self._last["rabi"] = key
+
+    def set_potential(self, V: torch.Tensor | None) -> None:
+        lib = _lib.load()
+        if V is None:
+            _lib.check(lib.nlse_plan_set_potential(self._h, None, 0, 0), "set_potential")
+            self._keep.pop("V", None)
+            return
+        if V.dtype in (torch.float64,):
+            V = V.to(torch.float32)
+        if V.dtype == torch.complex128:
+            V = V.to(torch.complex64)
+        V = V.contiguous()
+        kind = 1 if V.dtype == torch.float32 else 2
+        plane = self.ny * self.nx
+        if V.numel() == plane:
+            stride = 0
+        elif V.numel() == plane * self.items:
+            stride = plane
+        else:
+            raise ValueError(f"V with {V.numel()} points does not match the grid {self.ny}x{self.nx}")
+        self._keep["V"] = V
+        _lib.check(lib.nlse_plan_set_potential(self._h, V.data_ptr(), kind, stride), "set_potential")
+
+    def set_drive(self, Fr0, Fr1, Ft0, Ft1, dz: float) -> None:
+        lib = _lib.load()
+        if Fr0 is None:
+            _lib.check(lib.nlse_plan_set_drive(self._h, None, None, None, None, 0, 0.0), "set_drive")
+            return
+        r0 = D.to_device(Fr0, torch.complex64)
+        r1 = D.to_device(Fr1, torch.complex64)
+        t0 = np.ascontiguousarray(np.asarray(Ft0, dtype=np.complex64).reshape(-1))
+        t1 = np.ascontiguousarray(np.asarray(Ft1, dtype=np.complex64).reshape(-1))
+        n = min(t0.size, t1.size)
+        self._keep["drive"] = (r0, r1, t0, t1)
+        _lib.check(lib.nlse_plan_set_drive(self._h, r0.data_ptr(), r1.data_ptr(), t0.ctypes.data, t1.ctypes.data,
+                                           n, float(dz)), "set_drive")
+
+    # -- execution -------------------------------------------------------
+    def run(self, A: torch.Tensor, nsteps: int, precision: int = 0, step0: int = 0) -> None:
+        if A.dtype != torch.complex64 or not A.is_contiguous() or not A.is_cuda:
+            raise TypeError("field must be a contiguous complex64 device tensor")
+        if A.numel() != self.items * self.ncomp * self.ny * self.nx:
+            raise ValueError("field size does not match the plan")
+        _lib.check(_lib.load().nlse_run(self._h, A.data_ptr(), int(nsteps), int(precision), int(step0),
+                                        D.stream_ptr()), "run")
+
+    def config(self) -> tuple[int, int]:
+        a, b = C.c_int(), C.c_int()
+        _lib.check(_lib.load().nlse_plan_config(self._h, C.byref(a), C.byref(b)), "plan_config")
+        return a.value, b.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.nlse_plan_destroy(h)
+            self._h = None

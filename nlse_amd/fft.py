@@ -1,0 +1,59 @@
+"""FFT plan object of the GPU path: replaces ``pyvkfft.cuda.VkFFTApp``
+(NLSE/nlse.py:207-218).  In-place complex64 transform over the last
+``ndim`` axes, batched over the leading ones; forward unnormalised, inverse
+scaled by 1/N (VkFFT ``norm=1``).  ``fft(A, A)`` / ``ifft(A, A)`` keep the
+VkFFTApp calling convention used at nlse.py:435-438."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+
+
+class HipFFTPlan:
+    def __init__(self, shape, dtype=np.complex64, ndim: int = 2):
+        shape = tuple(int(s) for s in shape)
+        if ndim < 1 or ndim > 3 or len(shape) < ndim:
+            raise ValueError(f"bad FFT plan: shape {shape}, ndim {ndim}")
+        self.shape0 = shape
+        self.ndim = ndim
+        self.dtype = np.dtype(dtype)
+        self.dims = shape[len(shape) - ndim:]
+        self.batch = int(np.prod(shape[:len(shape) - ndim])) if len(shape) > ndim else 1
+        lib = _lib.load()
+        dims = (C.c_int64 * ndim)(*self.dims)
+        h = C.c_void_p()
+        _lib.check(lib.nlse_fft_plan_create(ndim, dims, self.batch, C.byref(h)), "fft_plan_create")
+        self._h = h
+
+    def _exec(self, src: torch.Tensor, dst: torch.Tensor, inverse: int) -> None:
+        if src.dtype != torch.complex64:
+            raise TypeError("the HIP FFT computes in complex64")
+        if tuple(src.shape) != self.shape0:
+            raise ValueError(f"array shape {tuple(src.shape)} != plan shape {self.shape0}")
+        if dst.data_ptr() != src.data_ptr():
+            dst.copy_(src)
+        if not dst.is_contiguous():
+            raise ValueError("FFT operand must be contiguous")
+        _lib.check(_lib.load().nlse_fft_exec(self._h, dst.data_ptr(), inverse, D.stream_ptr()), "fft_exec")
+
+    def fft(self, src: torch.Tensor, dst: torch.Tensor | None = None) -> torch.Tensor:
+        dst = src if dst is None else dst
+        self._exec(src, dst, 0)
+        return dst
+
+    def ifft(self, src: torch.Tensor, dst: torch.Tensor | None = None) -> torch.Tensor:
+        dst = src if dst is None else dst
+        self._exec(src, dst, 1)
+        return dst
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.nlse_fft_plan_destroy(h)
+            self._h = None

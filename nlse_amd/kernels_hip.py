@@ -1,0 +1,181 @@
+"""Operator API of the GPU path, same names and arguments as the reference's
+``NLSE/kernels_gpu.py`` (bound to ``self._kernels`` by the ``backend``
+setter, nlse.py:167-173), running the HIP kernels of libnlse_hip.so on
+torch device tensors.
+
+All functions mutate their first argument(s) in place and return None.
+Physical parameters may be Python scalars or arrays broadcastable to the
+leading (non-transverse) dimensions of the field, e.g. shape (B, 1, 1) for
+a (B, NY, NX) field (README.md:147-159).  ``tdims`` is the number of
+trailing transverse dimensions (2 for 2-D fields).
+
+The single-component loss follows the reference CPU semantics
+(kernels_cpu.py:33,62: ``-alpha``) unless ``loss_saturates=True``
+(kernels_gpu.py:30,62: ``-alpha*sat``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+
+
+def _items(A: torch.Tensor, tdims: int) -> tuple[int, int]:
+    item_size = int(np.prod(A.shape[A.dim() - tdims:])) if tdims else 1
+    return A.numel() // max(item_size, 1), item_size
+
+
+def _param(x, lead_shape, items):
+    """Broadcast a scalar or (lead..., 1...) array to one value per item."""
+    if isinstance(x, torch.Tensor):
+        x = x.detach().cpu().numpy()
+    a = np.asarray(x, dtype=np.float64)
+    if a.size == 1:
+        return D.bparam(a.reshape(-1)[0], items)
+    # drop trailing singleton (transverse) axes, broadcast over leading axes
+    while a.ndim > len(lead_shape) and a.shape[-1] == 1:
+        a = a[..., 0]
+    try:
+        a = np.broadcast_to(a, lead_shape)
+    except ValueError as e:
+        raise ValueError(f"parameter of shape {np.shape(x)} does not broadcast to the field's "
+                         f"leading shape {lead_shape} (per-point parameters are not supported)") from e
+    return D.bparam(a.reshape(-1), items)
+
+
+def _vkind(V):
+    if V is None:
+        return None, 0
+    if not isinstance(V, torch.Tensor):
+        V = D.to_device(V)
+    if V.dtype == torch.float32:
+        return V.contiguous(), 1
+    if V.dtype == torch.complex64:
+        return V.contiguous(), 2
+    if V.dtype == torch.float64:
+        return V.to(torch.float32).contiguous(), 1
+    if V.dtype == torch.complex128:
+        return V.to(torch.complex64).contiguous(), 2
+    raise TypeError(f"unsupported V dtype {V.dtype}")
+
+
+def _check_field(A: torch.Tensor):
+    if not (isinstance(A, torch.Tensor) and A.is_cuda and A.dtype == torch.complex64 and A.is_contiguous()):
+        raise TypeError("field must be a contiguous complex64 device tensor")
+
+
+def square_mod(A: torch.Tensor, A_sq: torch.Tensor) -> None:
+    """A_sq = |A|^2 (kernels_gpu.py:179-190)."""
+    _check_field(A)
+    assert A_sq.dtype == torch.float32 and A_sq.is_contiguous() and A_sq.numel() == A.numel()
+    _lib.check(_lib.load().nlse_square_mod(A.data_ptr(), A_sq.data_ptr(), A.numel(), D.stream_ptr()),
+               "square_mod")
+
+
+def nl_prop(A, A_sq, dz, alpha, V, g, Isat, *, tdims: int = 2, loss_saturates: bool = False) -> None:
+    """A *= exp(dz*(-alpha[*sat] + 1j*g*A_sq*sat + 1j*V)) (kernels_gpu.py:4-35)."""
+    _check_field(A)
+    items, isz = _items(A, tdims)
+    lead = tuple(A.shape[:A.dim() - tdims])
+    ta, pa = _param(alpha, lead, items)
+    tg, pg = _param(g, lead, items)
+    ti, pi = _param(Isat, lead, items)
+    Vt, vk = _vkind(V)
+    if Vt is not None and Vt.numel() != isz:
+        raise ValueError("V must have the transverse shape of the field")
+    _lib.check(_lib.load().nlse_nl_prop(
+        A.data_ptr(), A_sq.data_ptr(), items, isz, float(dz), pa,
+        None if Vt is None else Vt.data_ptr(), vk, pg, pi, int(loss_saturates), D.stream_ptr()), "nl_prop")
+    del ta, tg, ti
+
+
+def nl_prop_without_V(A, A_sq, dz, alpha, g, Isat, *, tdims: int = 2, loss_saturates: bool = False) -> None:
+    """nl_prop with V = None (kernels_gpu.py:38-65)."""
+    nl_prop(A, A_sq, dz, alpha, None, g, Isat, tdims=tdims, loss_saturates=loss_saturates)
+
+
+def _comp_view_strides(A1: torch.Tensor, tdims: int):
+    """Items, item size and item stride of a component view such as
+    A[..., 0, :, :] (cnlse.py:200-201) of a contiguous field."""
+    isz = int(np.prod(A1.shape[A1.dim() - tdims:]))
+    items = A1.numel() // isz
+    if items > 1:
+        stride = A1.stride(A1.dim() - tdims - 1)
+        # all leading axes must collapse to one stride
+        exp = stride
+        for ax in range(A1.dim() - tdims - 1, -1, -1):
+            if A1.shape[ax] > 1 and A1.stride(ax) != exp:
+                raise ValueError("component view with non-uniform leading strides")
+            exp *= A1.shape[ax]
+    else:
+        stride = isz
+    # the transverse block itself must be contiguous
+    s = 1
+    for ax in range(A1.dim() - 1, A1.dim() - tdims - 1, -1):
+        if A1.shape[ax] > 1 and A1.stride(ax) != s:
+            raise ValueError("component view must be contiguous over the transverse axes")
+        s *= A1.shape[ax]
+    return items, isz, stride
+
+
+def nl_prop_c(A1, A_sq_1, A_sq_2, dz, alpha, V, g11, g12, Isat1, Isat2, *, tdims: int = 2) -> None:
+    """Two-component nonlinear step for one component (kernels_gpu.py:68-104);
+    A1, A_sq_1, A_sq_2 may be component views of (..., 2, NY, NX) arrays."""
+    assert A1.dtype == torch.complex64 and A1.is_cuda
+    items, isz, astr = _comp_view_strides(A1, tdims)
+    _, _, istr = _comp_view_strides(A_sq_1, tdims)
+    _, _, istr2 = _comp_view_strides(A_sq_2, tdims)
+    if istr2 != istr:
+        raise ValueError("A_sq_1 and A_sq_2 must have the same layout")
+    lead = tuple(A1.shape[:A1.dim() - tdims])
+    keep = [_param(x, lead, items) for x in (alpha, g11, g12, Isat1, Isat2)]
+    Vt, vk = _vkind(V)
+    _lib.check(_lib.load().nlse_nl_prop_c(
+        A1.data_ptr(), A_sq_1.data_ptr(), A_sq_2.data_ptr(), items, isz, astr, istr, float(dz), keep[0][1],
+        None if Vt is None else Vt.data_ptr(), vk, keep[1][1], keep[2][1], keep[3][1], keep[4][1],
+        D.stream_ptr()), "nl_prop_c")
+
+
+def nl_prop_without_V_c(A1, A_sq_1, A_sq_2, dz, alpha, g11, g12, Isat1, Isat2, *, tdims: int = 2) -> None:
+    """nl_prop_c without potential (kernels_gpu.py:107-139)."""
+    nl_prop_c(A1, A_sq_1, A_sq_2, dz, alpha, None, g11, g12, Isat1, Isat2, tdims=tdims)
+
+
+def rabi_coupling(A1, A2, dz, omega, *, tdims: int = 2) -> None:
+    """Rabi rotation of the two components (kernels_gpu.py:142-156)."""
+    items, isz, str1 = _comp_view_strides(A1, tdims)
+    _, _, str2 = _comp_view_strides(A2, tdims)
+    if str1 != str2:
+        raise ValueError("A1 and A2 must have the same layout")
+    lead = tuple(A1.shape[:A1.dim() - tdims])
+    tw, pw = _param(omega, lead, items)
+    _lib.check(_lib.load().nlse_rabi_coupling(A1.data_ptr(), A2.data_ptr(), items, isz, str1, float(dz), pw,
+                                              D.stream_ptr()), "rabi_coupling")
+
+
+def mul_propagator(A: torch.Tensor, propagator: torch.Tensor) -> None:
+    """A *= propagator, broadcast over leading dims (nlse.py:437)."""
+    _check_field(A)
+    P = propagator if propagator.is_contiguous() else propagator.contiguous()
+    if P.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError("propagator must be complex64 or complex128")
+    kind = 1 if P.dtype == torch.complex64 else 2
+    pn = P.numel()
+    if A.numel() % pn != 0 or tuple(A.shape[A.dim() - P.dim():]) != tuple(P.shape):
+        raise ValueError(f"propagator of shape {tuple(P.shape)} does not broadcast to {tuple(A.shape)}")
+    # express both as items of the propagator's last-axis block
+    isz = P.shape[-1]
+    _lib.check(_lib.load().nlse_mul_propagator(A.data_ptr(), P.data_ptr(), kind, A.numel() // isz, isz,
+                                               pn // isz, D.stream_ptr()), "mul_propagator")
+
+
+def sumsq(A: torch.Tensor, tdims: int) -> torch.Tensor:
+    """Per-item FP64 sum of |A|^2 over the last ``tdims`` axes."""
+    _check_field(A)
+    items, isz = _items(A, tdims)
+    out = torch.empty(items, dtype=torch.float64, device=A.device)
+    _lib.check(_lib.load().nlse_sumsq(A.data_ptr(), items, isz, out.data_ptr(), D.stream_ptr()), "sumsq")
+    return out.reshape(A.shape[:A.dim() - tdims])

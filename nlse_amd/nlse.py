@@ -1,0 +1,447 @@
+"""NLSE: 2-D split-step solver on the MI355X engine.
+
+Drop-in for ``NLSE.NLSE`` of Quantum-Optics-LKB/NLSE v2.3.0 with
+``backend="GPU"`` (NLSE/nlse.py:41-619): same constructor, attributes,
+private helpers used by the reference tests, ``split_step`` and
+``out_field`` signatures, callbacks and leading-dimension broadcasting.
+Arrays that the reference keeps as CuPy arrays during ``out_field`` are
+torch device tensors here.  All arithmetic runs in libnlse_hip.so.
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Any, Callable, Union
+
+import numpy as np
+import torch
+from scipy import special
+from scipy.constants import c, epsilon_0
+
+from . import _device as D
+from . import _lib
+from . import kernels_hip
+from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, Plan2D, inv_or_zero
+from .fft import HipFFTPlan
+from .utils import __BACKEND__
+
+try:  # progress bar of the reference loop (nlse.py:551-559); optional
+    import tqdm
+except ImportError:  # pragma: no cover
+    tqdm = None
+
+
+def count_steps(z: float, delta_z) -> int:
+    """Iterations of the reference loop ``while abs(z_prop) < z: ...;
+    z_prop += delta_z`` (nlse.py:566-585) for a fixed delta_z."""
+    z_prop = 0
+    n = 0
+    while abs(z_prop) < z:
+        z_prop += delta_z
+        n += 1
+    return n
+
+
+def host(x):
+    """Parameter value on the host (numpy or Python scalar)."""
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return x
+
+
+def bcast_items(x, lead: tuple) -> np.ndarray:
+    """Scalar or (lead..., 1, 1)-shaped parameter -> float64[prod(lead)]."""
+    a = np.asarray(host(x), dtype=np.float64)
+    while a.ndim > len(lead):
+        if a.shape[-1] != 1:
+            raise ValueError(f"parameter of shape {np.shape(host(x))} varies over the transverse axes; "
+                             "only per-realisation parameters (shape (..., 1, 1)) are supported")
+        a = a[..., 0]
+    return np.ascontiguousarray(np.broadcast_to(a, lead)).reshape(-1)
+
+
+class NLSE:
+    """A class to solve NLSE (reference NLSE/nlse.py:41)."""
+
+    __CUPY_AVAILABLE__ = False
+    __PYOPENCL_AVAILABLE__ = False
+    __HIP_AVAILABLE__ = _lib.available()
+
+    def __init__(
+        self,
+        alpha: float,
+        power: float,
+        window: Union[float, tuple, list],
+        n2: float,
+        V: Union[np.ndarray, None],
+        L: float,
+        NX: int = 1024,
+        NY: int = 1024,
+        Isat: float = np.inf,
+        nl_length: float = 0,
+        wvl: float = 780e-9,
+        backend: str = __BACKEND__,
+    ) -> None:
+        """Instantiate the simulation (nlse.py:47-155).
+
+        Solves dz psi = -1/2k0 (d2x + d2y) psi + k0 dn psi + k0 n2 |psi|^2 psi.
+        Extra engine options are plain attributes set after construction:
+        ``loss_saturates`` (False: the reference CPU loss ``-alpha``; True:
+        the reference GPU loss ``-alpha*sat``) and ``col_width`` (column
+        tile of the column pass, 0 = default).
+        """
+        self.backend = backend
+        self.n2 = n2
+        self.V = V
+        self.wl = wvl
+        self.k = 2 * np.pi / self.wl
+        self.L = L
+        self.alpha = alpha
+        self.power = power
+        self.I_sat = Isat
+        self.NX = NX
+        self.NY = NY
+        if isinstance(window, float) or isinstance(window, int):
+            self.window = [window, window]
+        elif isinstance(window, tuple) or isinstance(window, list) or isinstance(window, np.ndarray):
+            self.window = window
+        Dn = self.n2 * self.power / min(self.window) ** 2
+        z_nl = 1 / (self.k * abs(Dn))
+        if isinstance(z_nl, np.ndarray):
+            z_nl = float(z_nl.min())
+        self.delta_z = 5e-3 * z_nl
+        self.X, self.delta_X = np.linspace(-self.window[0] / 2, self.window[0] / 2, num=NX,
+                                           endpoint=False, retstep=True, dtype=np.float32)
+        self.Y, self.delta_Y = np.linspace(-self.window[1] / 2, self.window[1] / 2, num=NY,
+                                           endpoint=False, retstep=True, dtype=np.float32)
+        self._last_axes = (-2, -1)
+        self.XX, self.YY = np.meshgrid(self.X, self.Y)
+        self.Kx = 2 * np.pi * np.fft.fftfreq(self.NX, d=self.delta_X)
+        self.Ky = 2 * np.pi * np.fft.fftfreq(self.NY, d=self.delta_Y)
+        self.Kxx, self.Kyy = np.meshgrid(self.Kx, self.Ky)
+        self._propagator = None
+        self._prop_factors = None  # (propagator object, px, py) when separable
+        self.plans = None
+        self.nl_length = nl_length
+        if self.nl_length > 0:
+            d = self.nl_length // self.delta_X
+            x = np.arange(-3 * d, 3 * d + 1)
+            y = np.arange(-3 * d, 3 * d + 1)
+            XX, YY = np.meshgrid(x, y)
+            R = np.hypot(XX, YY)
+            self.nl_profile = special.kn(0, R / d)
+            self.nl_profile[self.nl_profile.shape[0] // 2, self.nl_profile.shape[1] // 2] = np.nanmax(
+                self.nl_profile[np.logical_not(np.isinf(self.nl_profile))])
+            self.nl_profile /= self.nl_profile.sum()
+        else:
+            self.nl_profile = np.ones((self.NY, self.NX), dtype=np.float32)
+        # engine options
+        self.loss_saturates = False
+        self.col_width = 0
+        self._engine = None
+        self._engine_key = None
+
+    # ------------------------------------------------------------ backend
+    @property
+    def backend(self) -> str:
+        return self.__backend
+
+    @backend.setter
+    def backend(self, value: str) -> None:
+        """nlse.py:162-186: only the GPU binding exists here."""
+        if value not in ["CPU", "GPU", "CL"]:
+            raise ValueError("Backend must be 'CPU', 'GPU' or 'CL'")
+        if value != "GPU":
+            raise NotImplementedError(
+                f"backend {value!r} is the reference's own path; nlse_amd implements backend='GPU' "
+                "on libnlse_hip.so only")
+        D.require_gpu()
+        self.__backend = "GPU"
+        self._kernels = kernels_hip
+
+    # --------------------------------------------------------- propagator
+    @property
+    def propagator(self):
+        return self._propagator
+
+    @propagator.setter
+    def propagator(self, value) -> None:
+        # a propagator assigned from outside is used as given (dense table)
+        self._propagator = value
+        self._prop_factors = None
+
+    def _build_propagator(self) -> np.ndarray:
+        """nlse.py:188-197: complex64, built in FP64 on the host."""
+        return np.exp(-1j * 0.5 * (self.Kxx**2 + self.Kyy**2) / self.k * self.delta_z).astype(np.complex64)
+
+    def _separable_factors(self):
+        """(px, py) complex128 with propagator = complex64(px[None,:]*py[:,None])
+        for the current delta_z, or None if the propagator is not separable."""
+        px = np.exp(-1j * 0.5 * self.Kx**2 / self.k * self.delta_z)
+        py = np.exp(-1j * 0.5 * self.Ky**2 / self.k * self.delta_z)
+        return px[None, :], py[None, :]
+
+    def _own_propagator(self) -> None:
+        """Build and cache the propagator like out_field does (nlse.py:536-537),
+        remembering its separable factors for the fused column pass."""
+        prop = self._build_propagator()
+        self._propagator = prop
+        f = self._separable_factors()
+        self._prop_factors = None if f is None else (prop, f[0], f[1])
+
+    def _set_propagator_internal(self, value) -> None:
+        """Replace the stored object (host <-> device) keeping provenance."""
+        if self._prop_factors is not None and self._prop_factors[0] is self._propagator:
+            self._prop_factors = (value,) + self._prop_factors[1:]
+        self._propagator = value
+
+    # ------------------------------------------------------------- arrays
+    def _build_fft_plan(self, A) -> list:
+        """nlse.py:199-218 (GPU branch): one in-place plan for both directions."""
+        return [HipFFTPlan(tuple(A.shape), np.complex64, ndim=len(self._last_axes))]
+
+    def _field_to_device(self, E_in) -> torch.Tensor:
+        dt = E_in.dtype
+        if dt in (np.complex128, torch.complex128):
+            raise TypeError("the MI355X engine propagates complex64 fields; pass E_in as complex64")
+        if dt not in (np.complex64, torch.complex64):
+            raise TypeError("Type mismatch, E_in should be complex64")
+        return D.to_device(E_in)
+
+    def _lead(self, A) -> tuple:
+        return tuple(A.shape[: len(A.shape) - len(self._last_axes)])
+
+    def _cell(self) -> float:
+        return self.delta_X * self.delta_Y
+
+    def _norm_scale(self, E: torch.Tensor, target, with_ceps: bool, lead: tuple) -> np.ndarray:
+        """E_00 = sqrt(target / (sum|E|^2 * cell [* c eps0/2])) per item,
+        the integral accumulated in FP64 on the device (nlse.py:296-303)."""
+        s = kernels_hip.sumsq(E, len(self._last_axes)).cpu().numpy().reshape(-1)
+        integral = s * self._cell()
+        if with_ceps:
+            integral = integral * c * epsilon_0 / 2
+        return np.sqrt(bcast_items(target, lead) / integral)
+
+    def _scale_items(self, A: torch.Tensor, scale: np.ndarray, items: int, isz: int, stride: int) -> None:
+        t, bp = D.bparam(scale, items)
+        _lib.check(_lib.load().nlse_scale(A.data_ptr(), items, isz, stride, bp, D.stream_ptr()), "scale")
+        del t
+
+    def _prepare_output_array(self, E_in, normalize: bool):
+        """nlse.py:257-307 (GPU branch): A and A_sq on the device, A
+        normalised to self.power per realisation."""
+        E = self._field_to_device(E_in)
+        A = torch.empty_like(E)
+        A_sq = torch.zeros(E.shape, dtype=torch.float32, device=E.device)
+        A.copy_(E)
+        if normalize:
+            lead = self._lead(E)
+            scale = self._norm_scale(E, self.power, True, lead)
+            isz = int(np.prod(E.shape[len(lead):]))
+            self._scale_items(A, scale, max(1, int(np.prod(lead))), isz, isz)
+        return A, A_sq
+
+    def _send_arrays_to_gpu(self) -> None:
+        """nlse.py:309-327: V, nl_profile, propagator and array parameters
+        become device tensors for the duration of out_field."""
+        if self.V is not None and not D.is_device_tensor(self.V):
+            self.V = D.to_device(self.V)
+        if not D.is_device_tensor(self.nl_profile):
+            self.nl_profile = D.to_device(self.nl_profile)
+        if self._propagator is not None and not D.is_device_tensor(self._propagator):
+            self._set_propagator_internal(D.to_device(self._propagator))
+        for name in ("power", "n2", "alpha", "I_sat"):
+            val = getattr(self, name)
+            if isinstance(val, np.ndarray):
+                setattr(self, name, D.to_device(val))
+
+    def _retrieve_arrays_from_gpu(self) -> None:
+        """nlse.py:344-361."""
+        if self.V is not None:
+            self.V = D.to_host(self.V)
+        self.nl_profile = D.to_host(self.nl_profile)
+        if self._propagator is not None:
+            self._set_propagator_internal(D.to_host(self._propagator))
+        for name in ("power", "n2", "alpha", "I_sat"):
+            val = getattr(self, name)
+            if isinstance(val, torch.Tensor):
+                setattr(self, name, D.to_host(val))
+
+    # ------------------------------------------------------------- engine
+    def _ncomp(self) -> int:
+        return 1
+
+    def _engine_for(self, A: torch.Tensor) -> Plan2D:
+        lead = self._lead(A)
+        ncomp = self._ncomp()
+        items = int(np.prod(lead)) // ncomp if lead else 1
+        key = (items, ncomp, self.NY, self.NX, bool(self.loss_saturates), int(self.col_width))
+        if self._engine is None or self._engine_key != key:
+            self._engine = Plan2D(self.NY, self.NX, items, ncomp, self.loss_saturates, self.col_width)
+            self._engine_key = key
+        return self._engine
+
+    def _nl_records(self, dz: float, items_lead: tuple) -> np.ndarray:
+        """Per-realisation parameters of nl_prop[_without_V] for one
+        application with step dz (nlse.py:410-487): alpha/2,
+        g = k/2 n2 c eps0, Isat' = 2 I_sat/(eps0 c), V' = k/2 V."""
+        alpha = bcast_items(self.alpha, items_lead)
+        n2 = bcast_items(self.n2, items_lead)
+        isat = bcast_items(self.I_sat, items_lead)
+        rec = np.zeros(alpha.size, NL1_DTYPE)
+        rec["adz"] = dz * (alpha / 2)
+        rec["gdz"] = dz * (self.k / 2 * n2 * c * epsilon_0)
+        rec["inv_isat"] = inv_or_zero(2 * isat / (epsilon_0 * c))
+        rec["vdz"] = dz * (self.k / 2)
+        return rec
+
+    def _configure(self, eng: Plan2D, A: torch.Tensor, V, propagator, precision: str) -> None:
+        lead = self._lead(A)
+        dz = self.delta_z
+        if precision == "double":
+            half = self._nl_records(dz / 2, lead)
+            eng.set_nl1(SLOT_HALF_PRE, half)
+            eng.set_nl1(SLOT_HALF_POST, half)
+        else:
+            eng.set_nl1(SLOT_FULL, self._nl_records(dz, lead))
+        self._configure_tables(eng, V, propagator)
+
+    def _configure_tables(self, eng: Plan2D, V, propagator) -> None:
+        if self.nl_length > 0:
+            raise NotImplementedError("nonlocal response is not available in this build of the engine")
+        pf = self._prop_factors
+        if pf is not None and pf[0] is propagator:
+            eng.set_propagator_sep(pf[1], pf[2])
+        else:
+            eng.set_propagator_c64(D.to_device(propagator, torch.complex64))
+        eng.set_potential(None if V is None else D.to_device(V))
+
+    def _precision_flag(self, precision: str) -> int:
+        if precision not in ("single", "double"):
+            raise ValueError("precision must be 'single' or 'double'")
+        return 1 if precision == "double" else 0
+
+    # --------------------------------------------------------------- step
+    def split_step(self, A, A_sq, V, propagator, plans: list, precision: str = "single") -> None:
+        """One split step in place on A (nlse.py:372-487), real space to real
+        space: [N(dz/2)] F P F^-1 N(dz or dz/2), fused into three sweeps.
+        A_sq is accepted for signature compatibility; the intensity lives in
+        registers inside the fused passes and A_sq is not written."""
+        eng = self._engine_for(A)
+        self._configure(eng, A, V, propagator, precision)
+        eng.run(A, 1, self._precision_flag(precision), 0)
+
+    def _run_chunk(self, A, V, nsteps: int, precision: str, step0: int) -> None:
+        eng = self._engine_for(A)
+        self._configure(eng, A, V, self.propagator, precision)
+        eng.run(A, nsteps, self._precision_flag(precision), step0)
+
+    def out_field(
+        self,
+        E_in,
+        z: float,
+        plot: bool = False,
+        precision: str = "single",
+        verbose: bool = True,
+        normalize: bool = True,
+        callback: Union[list[Callable], Callable] = None,
+        callback_args: Union[list[tuple], tuple] = (),
+    ):
+        """Propagate the field at a distance z (nlse.py:489-619).
+
+        Without callbacks the whole loop is one engine call (two sweeps per
+        step, no host round trip); with callbacks each step runs and returns
+        to the host, and the callbacks get the device field as a torch tensor.
+        """
+        assert tuple(E_in.shape[self._last_axes[0]:]) == tuple(self.XX.shape[self._last_axes[0]:]), \
+            "Shape mismatch"
+        assert E_in.dtype in [np.complex64, np.complex128, torch.complex64, torch.complex128], \
+            "Type mismatch, E_in should be complex64 or complex128"
+        self._precision_flag(precision)
+        if self.propagator is None:
+            self._own_propagator()
+        self._send_arrays_to_gpu()
+        V = None if self.V is None else self.V.clone()
+        A, A_sq = self._prepare_output_array(E_in, normalize)
+        self.plans = self._build_fft_plan(A)
+        n2_old = self.n2
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        t0 = time.perf_counter()
+        if isinstance(self.delta_z, complex):
+            raise NotImplementedError("imaginary-time evolution (complex delta_z) is not supported")
+        if callback is None:
+            nsteps = count_steps(z, self.delta_z)
+            if nsteps and z > self.L:
+                self.n2 = 0
+            self._run_chunk(A, V, nsteps, precision, 0)
+        else:
+            self._loop_with_callbacks(A, A_sq, V, z, precision, verbose, callback, callback_args)
+        end.record()
+        end.synchronize()
+        t_gpu = start.elapsed_time(end)
+        if verbose:
+            print(f"\nTime spent to solve : {t_gpu * 1e-3} s (GPU) / {time.perf_counter() - t0} s (CPU)\n")
+        self.n2 = n2_old
+        return_np_array = isinstance(E_in, np.ndarray)
+        if return_np_array:
+            A = A.cpu().numpy()
+        self._retrieve_arrays_from_gpu()
+        if plot:
+            self.plot_field(A, z)
+        return A
+
+    def _loop_with_callbacks(self, A, A_sq, V, z, precision, verbose, callback, callback_args) -> None:
+        """The reference loop body (nlse.py:566-588), one step at a time."""
+        pbar = None
+        if verbose and tqdm is not None:
+            pbar = tqdm.tqdm(total=100, position=4, desc="Propagation", leave=False, unit="%",
+                             unit_scale=True)
+        z_prop = 0
+        i = 0
+        while abs(z_prop) < z:
+            if z > self.L:
+                self.n2 = 0
+            self.split_step(A, A_sq, V, self.propagator, self.plans, precision)
+            if isinstance(callback, Callable):
+                callback(self, A, z, i, *callback_args)
+            elif isinstance(callback, list) and isinstance(callback[0], Callable):
+                for cb, ca in zip(callback, callback_args):
+                    cb(self, A, z, i, *ca)
+            else:
+                raise ValueError("callbacks should be a callable or a list of callables")
+            z_prop += self.delta_z
+            i += 1
+            if pbar is not None:
+                pbar.n = abs(z_prop) / z * 100
+                pbar.refresh()
+        if pbar is not None:
+            pbar.close()
+
+    def plot_field(self, A_plot, z: float) -> None:
+        """Diagnostics figure (nlse.py:621-681); host-side, needs matplotlib."""
+        import matplotlib.pyplot as plt
+
+        A_plot = D.to_host(A_plot)
+        while A_plot.ndim > 2:
+            A_plot = A_plot[0]
+        fig, ax = plt.subplots(1, 3, layout="constrained", figsize=(15, 5))
+        fig.suptitle(rf"Field at $z$ = {z:.2e} m")
+        ext_real = [np.min(self.X) * 1e3, np.max(self.X) * 1e3, np.min(self.Y) * 1e3, np.max(self.Y) * 1e3]
+        ext_fourier = [np.min(self.Kx) * 1e-3, np.max(self.Kx) * 1e-3, np.min(self.Ky) * 1e-3,
+                       np.max(self.Ky) * 1e-3]
+        rho = np.abs(A_plot) ** 2 * 1e-4 * c / 2 * epsilon_0
+        phi = np.angle(A_plot)
+        im_fft = np.abs(np.fft.fftshift(np.fft.fft2(A_plot)))
+        im0 = ax[0].imshow(rho, extent=ext_real)
+        ax[0].set_title("Intensity")
+        fig.colorbar(im0, ax=ax[0], shrink=0.6, label=r"Intensity ($W/cm^2$)")
+        im1 = ax[1].imshow(phi, extent=ext_real, cmap="twilight_shifted", vmin=-np.pi, vmax=np.pi)
+        ax[1].set_title("Phase")
+        fig.colorbar(im1, ax=ax[1], shrink=0.6, label="Phase (rad)")
+        im2 = ax[2].imshow(im_fft, extent=ext_fourier, cmap="nipy_spectral")
+        ax[2].set_title("Fourier space")
+        fig.colorbar(im2, ax=ax[2], shrink=0.6, label="Intensity (a.u.)")
+        plt.show()
