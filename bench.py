@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: split-steps/s of the fused 2-D split step at 2048^2 complex64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one split step (linear step in Fourier space + real-space
+nonlinear step) of the 2048x2048 complex64 field of the config-1 physics
+(alpha=20, n2=-1.6e-9, Isat=1e5, real potential V, dz=5e-6, speckle input),
+with the field resident in HBM.  The run is the fused chain of the engine:
+K steps = one column sweep and one row sweep per step.
+
+Printed: one JSON line with the throughput, the roofline of the dominant
+kernel (its average duration measured with HIP events on the engine's
+stream), the step-level roofline, and the CPU baseline (the oracle's
+complex64 path on the host cores, a bounded sample of the same workload).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "split_step_2048_c64.md")
+
+
+def measured_traffic(sweep: str, grid: int):
+    """HBM bytes per launch of a sweep from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (2048^2 workload only), else None."""
+    if grid != 2048 or not os.path.exists(PROFILE):
+        return None
+    try:
+        text = open(PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)[sweep]["traffic_bytes"])
+    except (IndexError, KeyError, ValueError):
+        return None
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(N: int, budget_s: float, threads: int) -> dict:
+    """Oracle (complex64, scipy single-precision FFT, elementwise steps on a
+    thread pool) on the host cores: steps of the same 2048^2 workload until
+    `budget_s` of wall time has passed."""
+    import oracle
+    from nlse_amd import workloads as W
+
+    sim = W.config1(oracle.NLSE, N, nonlocal_length=0, fft="pocketfft32", workers=threads, threads=threads)
+    E0 = W.config1_field(sim)
+    A, A_sq = sim._prepare_output_array(E0, True)
+    prop = sim._build_propagator()
+    V = np.array(sim.V, copy=True)
+    sim.split_step(A, A_sq, V, prop, "single")  # warm-up (planning, pools)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        sim.split_step(A, A_sq, V, prop, "single")
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s and n >= 2:
+            break
+    return {
+        "value": n / el,
+        "unit": "split-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} split steps of the {N}x{N} config-1 workload (without the nonlocal term) on the "
+                  f"CPU oracle: complex64, scipy.fft single precision with {threads} workers, elementwise "
+                  f"steps on {threads} threads; {el:.1f} s; host CPU: {cpu_model()} "
+                  f"(os.cpu_count()={os.cpu_count()})",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every GPU box here has one "
+              "GPU; the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from nlse_amd import NLSE
+    from nlse_amd import workloads as W
+
+    N = args.grid
+    sim = W.config1(NLSE, N, nonlocal_length=0)
+    sim.col_width = args.col_width
+    E0 = W.config1_field(sim)
+    sim._own_propagator()
+    sim._send_arrays_to_gpu()
+    A, _ = sim._prepare_output_array(E0, True)
+    eng = sim._engine_for(A)
+    sim._configure(eng, A, sim.V, sim.propagator, "single")
+    P = A.numel()
+    vbytes = 4  # float32 V read once per row sweep
+    bytes_row = P * (16 + vbytes)
+    bytes_col = P * 16
+    step_bytes = P * (32 + vbytes)
+
+    # warm-up, then K steps bracketed by synchronisation
+    eng.run(A, args.warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    eng.run(A, args.steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_steps = ev0.elapsed_time(ev1) * 1e-3
+    steps_per_s = args.steps / t_steps
+    finite = bool(torch.isfinite(torch.view_as_real(A)).all())
+
+    # each sweep alone, same stream, same launch configuration
+    scratch = A.clone()
+    kern = {}
+    for name, which, nbytes in (("row", 0, bytes_row), ("col", 1, bytes_col)):
+        eng.run_pass(scratch, which, 10)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        eng.run_pass(scratch, which, args.steps)
+        e1.record()
+        torch.cuda.synchronize()
+        avg = e0.elapsed_time(e1) * 1e-3 / args.steps
+        kern[name] = {"avg_us": avg * 1e6, "bytes": nbytes, "GBps": nbytes / avg / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    achieved = kern[dom]["bytes"] / (kern[dom]["avg_us"] * 1e-6)
+    row_lines, col_width = eng.config()
+
+    result = {
+        "metric": "split-steps/s at 2048^2 c64",
+        "value": steps_per_s,
+        "unit": "split-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_steps / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"config-1 physics without the nonlocal term: {N}x{N} complex64 NLSE split step, "
+                        "real float32 potential, Isat=1e5, alpha=20, dz=5e-6, speckle input (seed 1), "
+                        "precision='single'",
+            "grid": [N, N],
+            "kernels": {"row_lines_per_block": row_lines, "col_width": col_width},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"{dom} sweep",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": measured_traffic(dom, N),
+        },
+        "kernels": kern,
+        "step_roofline": {
+            "bytes_per_step": step_bytes,
+            "B_per_point": step_bytes / P,
+            "achieved_GBps": steps_per_s * step_bytes / 1e9,
+            "frac": steps_per_s * step_bytes / HBM_PEAK,
+        },
+        "grid_point_steps_per_s": steps_per_s * P,
+        "field_finite": finite,
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -175,6 +175,11 @@ int nlse_plan_set_drive(nlse_plan* plan, const void* Fr0_dev, const void* Fr1_de
  * index of the first step (drive time index). */
 int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t step0,
              void* stream);
+/* Launch one sweep type `count` times in place on A (device): pass 0 is a
+ * steady-state row sweep (I_x, nonlinear step, F_x), pass 1 a column sweep
+ * (F_y, propagator, I_y).  The data is meaningless afterwards; this exists
+ * to time each kernel of the step in isolation. */
+int nlse_run_pass(nlse_plan* plan, void* A_dev, int pass, int64_t count, int precision, void* stream);
 /* Kernel configuration actually used (rows per block, columns per block). */
 int nlse_plan_config(const nlse_plan* plan, int* row_lines, int* col_width);
 

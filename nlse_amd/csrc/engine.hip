@@ -14,6 +14,10 @@
 #include "../../include/nlse_hip.h"
 #include "passes_2d.hpp"
 
+#ifndef NLSE_COL_LDS_KB
+#define NLSE_COL_LDS_KB 150  // LDS budget of one column-sweep block
+#endif
+
 using namespace nlse;
 
 // ------------------------------------------------------------ errors
@@ -97,15 +101,28 @@ template <int N>
 struct LineCfg {
   static constexpr int P = N >= 16 ? 16 : N;
   static constexpr int T = N / P;
-  static constexpr int LINE = Plan<N, P>::LDS_LINE;
+  static constexpr int LINE = LdsLine<N, 3, 3>::SIZE;  // column-pass lines
   // rows per block: 256 threads for one component, 128 for pairs
   static constexpr int LPB1 = (256 / T) >= 1 ? (256 / T) : 1;
   static constexpr int LPB2 = (128 / T) >= 1 ? (128 / T) : 1;
-  // widest column tile: <= 512 threads (256 VGPRs per lane) and <= 144 KiB of LDS, <= 64
-  static constexpr int WT = 512 / T;
-  static constexpr int WL = (144 * 1024) / (LINE * 8);
-  static constexpr int WM0 = WT < WL ? WT : WL;
-  static constexpr int WMAX = WM0 > 64 ? 64 : WM0;
+  // default column tile: 8 columns (64-byte row segments; wider tiles
+  // spill registers at 512 threads)
+  static constexpr int WDEF = 8;
+};
+
+// Column tile of W columns: WT threads across (<= 512 threads per block,
+// i.e. up to 256 VGPRs per lane), G = W/WT columns per thread, LDS rounds
+// of GR columns per thread column (<= 150 KiB of LDS).
+template <int N, int W>
+struct ColCfg {
+  using C = LineCfg<N>;
+  static constexpr int WTMAX = (512 / C::T) >= 1 ? (512 / C::T) : 1;
+  static constexpr int WT = W < WTMAX ? W : WTMAX;
+  static constexpr int G = W / WT;
+  static constexpr int GRMAX = (NLSE_COL_LDS_KB * 1024) / (WT * C::LINE * 8);
+  static constexpr int gr(int g) { return (g <= GRMAX || g == 1) ? g : gr(g / 2); }
+  static constexpr int GR = gr(G);
+  static constexpr bool OK = G >= 1 && G <= 4 && GR >= 1 && WT * GR * C::LINE * 8 <= 160 * 1024;
 };
 
 template <int N>
@@ -124,35 +141,55 @@ static int row_launch(int G, const RowArgs& a, hipStream_t s) {
 }
 
 template <int N, int W>
-static void col_launch_w(const ColArgs& a, hipStream_t s) {
+static bool col_launch_w(const ColArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
-  if constexpr (W <= C::WMAX) {
+  using K = ColCfg<N, W>;
+  if constexpr (K::OK) {
     const int64_t blocks = a.nslabs * (a.NX / W);
-    hipLaunchKernelGGL((k_col<N, C::P, W>), dim3((unsigned)blocks), dim3(W * C::T), 0, s, a);
+    hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR>), dim3((unsigned)blocks), dim3(K::WT * C::T), 0, s,
+                       a);
+    return true;
+  } else {
+    return false;
+  }
+}
+
+template <int N>
+static bool col_ok(int w) {
+  switch (w) {
+    case 1: return ColCfg<N, 1>::OK;
+    case 2: return ColCfg<N, 2>::OK;
+    case 4: return ColCfg<N, 4>::OK;
+    case 8: return ColCfg<N, 8>::OK;
+    case 16: return ColCfg<N, 16>::OK;
+    case 32: return ColCfg<N, 32>::OK;
+    case 64: return ColCfg<N, 64>::OK;
+    default: return false;
   }
 }
 
 template <int N>
 static int col_width_for(int64_t ncols, int want) {
-  using C = LineCfg<N>;
-  int w = want > 0 ? want : C::WMAX;
-  if (w > C::WMAX) w = C::WMAX;
-  while (w > 1 && (w > ncols || (ncols % w) != 0)) w >>= 1;
+  int w = want > 0 ? want : LineCfg<N>::WDEF;
+  if (w > 64) w = 64;
+  while (w > 1 && (w > ncols || (ncols % w) != 0 || !col_ok<N>(w))) w >>= 1;
   return w;
 }
 
 template <int N>
 static int col_launch(int W, const ColArgs& a, hipStream_t s) {
+  bool ok = false;
   switch (W) {
-    case 1: col_launch_w<N, 1>(a, s); break;
-    case 2: col_launch_w<N, 2>(a, s); break;
-    case 4: col_launch_w<N, 4>(a, s); break;
-    case 8: col_launch_w<N, 8>(a, s); break;
-    case 16: col_launch_w<N, 16>(a, s); break;
-    case 32: col_launch_w<N, 32>(a, s); break;
-    case 64: col_launch_w<N, 64>(a, s); break;
-    default: return fail(NLSE_ERR_ARG, "unsupported column width %d", W);
+    case 1: ok = col_launch_w<N, 1>(a, s); break;
+    case 2: ok = col_launch_w<N, 2>(a, s); break;
+    case 4: ok = col_launch_w<N, 4>(a, s); break;
+    case 8: ok = col_launch_w<N, 8>(a, s); break;
+    case 16: ok = col_launch_w<N, 16>(a, s); break;
+    case 32: ok = col_launch_w<N, 32>(a, s); break;
+    case 64: ok = col_launch_w<N, 64>(a, s); break;
+    default: break;
   }
+  if (!ok) return fail(NLSE_ERR_ARG, "unsupported column width %d for length %d", W, N);
   return launch_status("k_col");
 }
 
@@ -718,25 +755,19 @@ int nlse_plan_set_drive(nlse_plan* p, const void* fr0, const void* fr1, const fl
   return NLSE_OK;
 }
 
-int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, void* stream) {
-  if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
-  if (nsteps == 0) return NLSE_OK;
+static int check_ready(const nlse_plan* p, int precision) {
   if (p->pkind < 0) return fail(NLSE_ERR_STATE, "propagator not set");
-  const int s_full = NLSE_SLOT_FULL, s_pre = NLSE_SLOT_HALF_PRE, s_post = NLSE_SLOT_HALF_POST;
-  if (precision == 0 && !p->nl[s_full]) return fail(NLSE_ERR_STATE, "nonlinear parameters (single) not set");
-  if (precision == 1 && (!p->nl[s_pre] || !p->nl[s_post]))
+  if (precision == 0 && !p->nl[NLSE_SLOT_FULL]) return fail(NLSE_ERR_STATE, "nonlinear parameters (single) not set");
+  if (precision == 1 && (!p->nl[NLSE_SLOT_HALF_PRE] || !p->nl[NLSE_SLOT_HALF_POST]))
     return fail(NLSE_ERR_STATE, "nonlinear parameters (double) not set");
-  if (p->drive && step0 + nsteps > (int64_t)p->ft0.size())
-    return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
-                (long long)(step0 + nsteps - 1));
-  hipStream_t s = (hipStream_t)stream;
-  const int G = p->d.ncomp;
-  const int64_t ny = p->d.ny, nx = p->d.nx;
+  return NLSE_OK;
+}
 
+static RowArgs plan_row_args(const nlse_plan* p, void* A) {
   RowArgs r = row_defaults();
   r.A = (float2*)A;
-  r.rows_per_item = ny;
-  r.ngroups = p->d.items * ny * (G == 2 ? 1 : 1);
+  r.rows_per_item = p->d.ny;
+  r.ngroups = p->d.items * p->d.ny;
   r.tw = p->twx;
   r.loss_sat = p->d.loss_saturates;
   r.V = p->V;
@@ -746,48 +777,88 @@ int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0
   r.drv_r0 = p->fr0;
   r.drv_r1 = p->fr1;
   r.drv_dz = p->drv_dz;
+  return r;
+}
 
+static ColArgs plan_col_args(const nlse_plan* p, void* A) {
   ColArgs c = col_defaults();
   c.A = (float2*)A;
-  c.nslabs = p->d.items * G;
-  c.NX = (int)nx;
-  c.ncomp = G;
+  c.nslabs = p->d.items * p->d.ncomp;
+  c.NX = (int)p->d.nx;
+  c.ncomp = p->d.ncomp;
   c.tw = p->twy;
   c.do_fwd = c.do_prop = c.do_inv = 1;
   c.pkind = p->pkind;
   c.px = p->px;
   c.py = p->py;
   c.ptab = p->ptab;
-  c.scale = (float)(1.0 / ((double)nx * (double)ny));
+  c.scale = (float)(1.0 / ((double)p->d.nx * (double)p->d.ny));
+  return c;
+}
 
+// steady-state row sweep of step k (not the first or the last sweep of a run)
+static RowArgs mid_row(const nlse_plan* p, const RowArgs& r, int precision, bool last) {
+  RowArgs a = r;
+  a.inverse_first = 1;
+  a.forward_last = last ? 0 : 1;
+  a.has_a = 1;
+  a.nlp_a = precision == 1 ? p->nl[NLSE_SLOT_HALF_POST] : p->nl[NLSE_SLOT_FULL];
+  a.has_rabi = (precision == 0 && p->rabi != nullptr && p->d.ncomp == 2) ? 1 : 0;
+  a.has_b = (precision == 1 && !last) ? 1 : 0;
+  a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
+  return a;
+}
+
+int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, void* stream) {
+  if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
+  if (nsteps == 0) return NLSE_OK;
+  int rc = check_ready(p, precision);
+  if (rc) return rc;
+  if (p->drive && step0 + nsteps > (int64_t)p->ft0.size())
+    return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
+                (long long)(step0 + nsteps - 1));
+  hipStream_t s = (hipStream_t)stream;
+  const int G = p->d.ncomp;
+  const RowArgs r = plan_row_args(p, A);
+  const ColArgs c = plan_col_args(p, A);
   // first row pass: F_x, preceded by the half step of "double"
   {
     RowArgs a = r;
-    a.inverse_first = 0;
     a.forward_last = 1;
     a.has_b = precision == 1;
-    a.nlp_b = p->nl[s_pre];
-    int rc = p->rops->row(G, a, s);
+    a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
+    rc = p->rops->row(G, a, s);
     if (rc) return rc;
   }
   for (int64_t k = 0; k < nsteps; ++k) {
-    int rc = p->cops->col(p->colw, c, s);
+    rc = p->cops->col(p->colw, c, s);
     if (rc) return rc;
-    const bool last = (k == nsteps - 1);
-    RowArgs a = r;
-    a.inverse_first = 1;
-    a.forward_last = last ? 0 : 1;
-    a.has_a = 1;
-    a.nlp_a = precision == 1 ? p->nl[s_post] : p->nl[s_full];
-    a.has_rabi = (precision == 0 && p->rabi != nullptr && G == 2) ? 1 : 0;
+    RowArgs a = mid_row(p, r, precision, k == nsteps - 1);
     a.has_drive = p->drive ? 1 : 0;
     if (p->drive) {
       a.drv_t0 = p->ft0[step0 + k];
       a.drv_t1 = p->ft1[step0 + k];
     }
-    a.has_b = (precision == 1 && !last) ? 1 : 0;
-    a.nlp_b = p->nl[s_pre];
     rc = p->rops->row(G, a, s);
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
+int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision, void* stream) {
+  if (!p || !A || count < 0 || (pass != 0 && pass != 1) || (precision != 0 && precision != 1))
+    return fail(NLSE_ERR_ARG, "bad run_pass arguments");
+  int rc = check_ready(p, precision);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // repeated sweeps of one kind: 1/N per sweep keeps the data at its scale
+  // (in a run the 1/(NX NY) of the propagator normalises both transforms)
+  RowArgs r = mid_row(p, plan_row_args(p, A), precision, false);
+  r.out_scale = (float)(1.0 / (double)p->d.nx);
+  ColArgs c = plan_col_args(p, A);
+  c.scale = (float)(1.0 / (double)p->d.ny);
+  for (int64_t k = 0; k < count; ++k) {
+    rc = pass == 0 ? p->rops->row(p->d.ncomp, r, s) : p->cops->col(p->colw, c, s);
     if (rc) return rc;
   }
   return NLSE_OK;

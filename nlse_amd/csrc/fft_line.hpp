@@ -18,11 +18,10 @@
 // after 1000 steps).  The internal radix-8/16 constants (1/sqrt2, cos pi/8,
 // sin pi/8) are therefore applied as hi+lo float pairs with FMAs, so the
 // only error left is round-to-nearest of data-dependent values.  The stage
-// twiddles are stored as hi+lo float pairs too (exact to ~2^-48) and applied
-// with a compensated complex multiply: a table of twiddles rounded to
-// complex64 has a mean |w|^2 - 1 of about -1e-8 per stage (its few short
-// stages are dominated by near-axis values whose cos rounds down), which a
-// forward/inverse pair turns into a -5e-8 gain per split step.
+// twiddles are hi+lo pairs as well (see cmul_tw): a table of twiddles
+// rounded to complex64 has a mean |w|^2 - 1 of about -1e-8 per stage (its
+// few short stages are dominated by near-axis values whose cos rounds
+// down), which a forward/inverse pair turns into a -5e-8 gain per step.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -57,7 +56,9 @@ NLSE_SPLIT(S8, 0.38268343236508977172845998403039887)   // sin(pi/8)
               fmaf((a).y, C##_HI, fmaf(-(a).x, S##_HI, fmaf((a).y, C##_LO, -(a).x * S##_LO))))
 
 // a * w for a stage twiddle w = (c_hi + c_lo) + i (s_hi + s_lo),
-// tw = {c_hi, s_hi, c_lo, s_lo}
+// tw = {c_hi, s_hi, c_lo, s_lo}.  The lo terms enter the FMA chain before
+// the final rounding: a correction applied after rounding (e.g. a
+// magnitude factor 1 + 1e-8) would be lost below half an ulp.
 __device__ __forceinline__ float2 cmul_tw(float2 a, float4 w) {
   return make_float2(fmaf(a.x, w.x, fmaf(-a.y, w.y, fmaf(a.x, w.z, -a.y * w.w))),
                      fmaf(a.x, w.y, fmaf(a.y, w.x, fmaf(a.x, w.w, a.y * w.z))));
@@ -177,16 +178,24 @@ struct Plan {
     return off;
   }
   static constexpr int TW_SIZE = tw_offset(NSTAGES);
-  // padded LDS line length
-  static constexpr int LDS_LINE = N + N / 16 + (N < 16 ? 1 : 0);
 };
 
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+// LDS line layout: one float2 of padding every 2^PADSH entries, plus EXTRA
+// float2 at the end of each line (staggers the lines of a multi-line
+// block across banks).
+template <int N, int PADSH, int EXTRA>
+struct LdsLine {
+  static constexpr int SIZE = N + (N >> PADSH) + EXTRA + 1;
+};
 
-// One Stockham stage on G lines owned by the same thread.
-//   FROM_LDS: inputs come from LDS (else from registers v)
-//   TO_LDS:   outputs go to LDS (else, last stage, back to v)
-template <int N, int P, int S, int G>
+template <int PADSH>
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> PADSH); }
+
+// One Stockham stage on G lines owned by the same thread.  Non-last stages
+// exchange through LDS in rounds of GR lines (the LDS region holds GR lines
+// per thread column: line g of round r lives at slot g - r*GR); the last
+// stage (Ns = N/R) writes its outputs back to the natural set in registers.
+template <int N, int P, int S, int G, int GR, int PADSH>
 __device__ __forceinline__ void fft_stage(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
                                           float2* lds, int lds_stride) {
   using PL = Plan<N, P>;
@@ -195,82 +204,82 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[G][P], int t, const float4
   constexpr int NB = P / R;  // butterflies per thread
   constexpr bool LAST = (S == PL::NSTAGES - 1);
   constexpr int T = PL::T;
+  static_assert(G % GR == 0, "lines per round must divide the lines per thread");
+  float4 tws[NB][R > 1 ? R - 1 : 1];
+  if constexpr (NS > 1) {
+    const float4* twst = tw + PL::tw_offset(S);
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * T;
-    const int jm = j & (NS - 1);
-    float4 tws[R > 1 ? R - 1 : 1];
-    if constexpr (NS > 1) {
-      const float4* twst = tw + PL::tw_offset(S);
+    for (int b = 0; b < NB; ++b) {
+      const int jm = (t + b * T) & (NS - 1);
 #pragma unroll
-      for (int k = 1; k < R; ++k) tws[k - 1] = twst[(k - 1) * NS + jm];
+      for (int k = 1; k < R; ++k) tws[b][k - 1] = twst[(k - 1) * NS + jm];
     }
+  }
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      float2 x[R];
+  for (int r = 0; r < G / GR; ++r) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) x[k] = v[g][b + k * NB];
-      if constexpr (NS > 1) {
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * T;
+      const int jm = j & (NS - 1);
 #pragma unroll
-        for (int k = 1; k < R; ++k) x[k] = cmul_tw(x[k], tws[k - 1]);
+      for (int gg = 0; gg < GR; ++gg) {
+        const int g = r * GR + gg;
+        float2 x[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = v[g][b + k * NB];
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int k = 1; k < R; ++k) x[k] = cmul_tw(x[k], tws[b][k - 1]);
+        }
+        dft<R>(x);
+        if constexpr (LAST) {
+#pragma unroll
+          for (int k = 0; k < R; ++k) v[g][b + k * NB] = x[k];
+        } else {
+          const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+          for (int k = 0; k < R; ++k) lds[gg * lds_stride + lds_pad<PADSH>(base + k * NS)] = x[k];
+        }
       }
-      dft<R>(x);
-      if constexpr (LAST) {
+    }
+    if constexpr (!LAST) {
+      __syncthreads();
 #pragma unroll
-        for (int k = 0; k < R; ++k) v[g][b + k * NB] = x[k];
-      } else {
-        const int base = (j / NS) * NS * R + jm;
+      for (int gg = 0; gg < GR; ++gg)
 #pragma unroll
-        for (int k = 0; k < R; ++k) lds[g * lds_stride + lds_pad(base + k * NS)] = x[k];
-      }
+        for (int m = 0; m < P; ++m) v[r * GR + gg][m] = lds[gg * lds_stride + lds_pad<PADSH>(t + m * T)];
+      __syncthreads();
     }
   }
 }
 
-template <int N, int P, int G>
-__device__ __forceinline__ void lds_gather(float2 (&v)[G][P], int t, const float2* lds, int lds_stride) {
-  constexpr int T = Plan<N, P>::T;
-#pragma unroll
-  for (int g = 0; g < G; ++g)
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[g][m] = lds[g * lds_stride + lds_pad(t + m * T)];
-}
-
-// Barrier between LDS phases.  WG-wide: the lines of a block may span waves.
-__device__ __forceinline__ void lds_sync() { __syncthreads(); }
-
-template <int N, int P, int S, int G>
+template <int N, int P, int S, int G, int GR, int PADSH>
 __device__ __forceinline__ void fft_stages_from(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
                                                 float2* lds, int lds_stride) {
-  using PL = Plan<N, P>;
-  if constexpr (S < PL::NSTAGES) {
-    fft_stage<N, P, S, G>(v, t, tw, lds, lds_stride);
-    if constexpr (S < PL::NSTAGES - 1) {
-      lds_sync();
-      lds_gather<N, P, G>(v, t, lds, lds_stride);
-      lds_sync();
-    }
-    fft_stages_from<N, P, S + 1, G>(v, t, tw, lds, lds_stride);
+  if constexpr (S < Plan<N, P>::NSTAGES) {
+    fft_stage<N, P, S, G, GR, PADSH>(v, t, tw, lds, lds_stride);
+    fft_stages_from<N, P, S + 1, G, GR, PADSH>(v, t, tw, lds, lds_stride);
   }
 }
 
 // Forward (INV=false) or unnormalised inverse (INV=true) DFT of G lines,
-// natural set in, natural set out.  The inverse is conj(F(conj(x))).
-template <int N, int P, int G, bool INV>
+// natural set in, natural set out.  The inverse is swap(F(swap(x))) with
+// swap(a + ib) = b + ia (= conj(F(conj x))), a register renaming.
+template <int N, int P, int G, bool INV, int GR = G, int PADSH = 4>
 __device__ __forceinline__ void fft_line(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
                                          float2* lds, int lds_stride) {
   if constexpr (INV) {
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[g][m].y = -v[g][m].y;
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].y, v[g][m].x);
   }
-  fft_stages_from<N, P, 0, G>(v, t, tw, lds, lds_stride);
+  fft_stages_from<N, P, 0, G, GR, PADSH>(v, t, tw, lds, lds_stride);
   if constexpr (INV) {
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[g][m].y = -v[g][m].y;
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].y, v[g][m].x);
   }
 }
 

@@ -51,7 +51,7 @@ template <int N, int P, int LPB, int G>
 __global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = PL::LDS_LINE;
+  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
   __shared__ float2 lds[LPB * G * LINE];
 
   const int li = threadIdx.x / T;
@@ -76,60 +76,75 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[off[g] + t + m * T] : make_float2(0.f, 0.f);
 
+  // side inputs of the real-space step, fetched with the field so that
+  // their latency overlaps the inverse transform
+  const bool has_nl = a.has_a || a.has_b;
+  float vr[P], vi[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    vr[m] = 0.f;
+    vi[m] = 0.f;
+  }
+  if (has_nl && a.vkind != V_NONE) {
+    const int64_t vbase = item * a.v_item_stride + y * N + t;
+#pragma unroll
+    for (int m = 0; m < P; ++m) load_v(a.V, a.vkind, vbase + m * T, vr[m], vi[m]);
+  }
+
   if (a.inverse_first) fft_line<N, P, G, true>(v, t, a.tw, lline, LINE);
 
-  // real-space operator on the natural set x = t + m*T
-  if (a.has_a || a.has_b || a.has_drive) {
-    const int64_t vbase = item * a.v_item_stride + y * N;
+  if constexpr (G == 1) {
+    if (a.has_a) {
+      const NLParams1 pa = reinterpret_cast<const NLParams1*>(a.nlp_a)[item];
+      nl1_block<P>(v[0], pa, a.loss_sat, vr, vi);
+    }
+    if (a.has_b) {
+      const NLParams1 pb = reinterpret_cast<const NLParams1*>(a.nlp_b)[item];
+      nl1_block<P>(v[0], pb, a.loss_sat, vr, vi);
+    }
+  } else {
+    if (a.has_a) {
+      const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_a) + 2 * item;
+      const NLParams2 p1 = p[0], p2 = p[1];
+      nl2_block<P>(v[0], v[1], p1, p2, vr, vi);
+      if (a.has_rabi) {
+        const RabiParams rb = a.rabi[item];
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-      const int x = t + m * T;
-      float vr, vi;
-      load_v(a.V, a.vkind, vbase + x, vr, vi);
-      if constexpr (G == 1) {
-        if (a.has_a) v[0][m] = nl1(v[0][m], reinterpret_cast<const NLParams1*>(a.nlp_a)[item], a.loss_sat, vr, vi);
-        if (a.has_b) v[0][m] = nl1(v[0][m], reinterpret_cast<const NLParams1*>(a.nlp_b)[item], a.loss_sat, vr, vi);
-      } else {
-        if (a.has_a) {
-          const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_a) + 2 * item;
-          const float I1 = v[0][m].x * v[0][m].x + v[0][m].y * v[0][m].y;
-          const float I2 = v[1][m].x * v[1][m].x + v[1][m].y * v[1][m].y;
-          v[0][m] = nl2(v[0][m], I1, I2, p[0], vr, vi);
-          v[1][m] = nl2(v[1][m], I2, I1, p[1], vr, vi);
-          if (a.has_rabi) rabi(v[0][m], v[1][m], a.rabi[item]);
-        }
-        if (a.has_drive) {
-          const int64_t ri = y * N + x;
-          float2 q = cmul(a.drv_r0[ri], a.drv_t0);
-          q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
-          v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
-          q = cmul(a.drv_r1[ri], a.drv_t1);
-          q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
-          v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
-        }
-        if (a.has_b) {
-          const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_b) + 2 * item;
-          const float I1 = v[0][m].x * v[0][m].x + v[0][m].y * v[0][m].y;
-          const float I2 = v[1][m].x * v[1][m].x + v[1][m].y * v[1][m].y;
-          v[0][m] = nl2(v[0][m], I1, I2, p[0], vr, vi);
-          v[1][m] = nl2(v[1][m], I2, I1, p[1], vr, vi);
-        }
+        for (int m = 0; m < P; ++m) rabi(v[0][m], v[1][m], rb);
       }
+    }
+    if (a.has_drive) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const int64_t ri = y * N + t + m * T;
+        float2 q = cmul(a.drv_r0[ri], a.drv_t0);
+        q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+        v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+        q = cmul(a.drv_r1[ri], a.drv_t1);
+        q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+        v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+      }
+    }
+    if (a.has_b) {
+      const NLParams2* p = reinterpret_cast<const NLParams2*>(a.nlp_b) + 2 * item;
+      const NLParams2 p1 = p[0], p2 = p[1];
+      nl2_block<P>(v[0], v[1], p1, p2, vr, vi);
     }
   }
 
   if (a.forward_last) fft_line<N, P, G, false>(v, t, a.tw, lline, LINE);
 
+  if (a.out_scale != 1.f) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].x * a.out_scale, v[g][m].y * a.out_scale);
+  }
   if (active) {
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-      for (int m = 0; m < P; ++m) {
-        float2 r = v[g][m];
-        r.x *= a.out_scale;
-        r.y *= a.out_scale;
-        a.A[off[g] + t + m * T] = r;
-      }
+      for (int m = 0; m < P; ++m) a.A[off[g] + t + m * T] = v[g][m];
   }
 }
 
@@ -150,47 +165,74 @@ struct ColArgs {
   float out_scale;     // applied to the stored result when no propagator
 };
 
-template <int N, int P, int W>
-__global__ __launch_bounds__(W*(N / P)) void k_col(ColArgs a) {
+// Column sweep.  A block covers W = WT*G adjacent columns of one slab:
+// WT threads across (consecutive lanes read consecutive columns), each
+// thread owning G columns (c0 + g*WT + cw) so that twiddles and the y
+// factor of the propagator are shared; LDS exchanges run in rounds of GR
+// columns per thread column.
+template <int N, int P, int WT, int G, int GR>
+__global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = PL::LDS_LINE;
-  __shared__ float2 lds[W * LINE];
+  // one float2 of padding every 8 entries and 4 more per line: the lines
+  // of the WT thread columns then start 8 banks apart
+  constexpr int LINE = LdsLine<N, 3, 3>::SIZE;
+  constexpr int W = WT * G;
+  __shared__ float2 lds[WT * GR * LINE];
 
-  const int cw = threadIdx.x % W;
-  const int t = threadIdx.x / W;
+  const int cw = threadIdx.x % WT;
+  const int t = threadIdx.x / WT;
   const int tiles = a.NX / W;
-  const int64_t slab = blockIdx.x / tiles;
-  const int c = (blockIdx.x % tiles) * W + cw;
-  const int64_t base = slab * (int64_t)N * a.NX + c;
-  float2* lline = lds + cw * LINE;
+  // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+  // so block b runs on XCD b % 8; give each XCD a contiguous range of
+  // tiles so that the two 64-byte halves of a 128-byte line are fetched
+  // by neighbouring tiles on the same XCD (one L2 fill instead of two)
+  const int64_t nblk = (int64_t)gridDim.x;
+  const int64_t b = (nblk % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int64_t slab = b / tiles;
+  const int c0 = (int)(b % tiles) * W + cw;
+  const int64_t base = slab * (int64_t)N * a.NX + c0;
+  float2* lline = lds + cw * GR * LINE;
 
-  float2 v[1][P];
+  float2 v[G][P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) v[0][m] = a.A[base + (int64_t)(t + m * T) * a.NX];
+  for (int m = 0; m < P; ++m)
+#pragma unroll
+    for (int g = 0; g < G; ++g) v[g][m] = a.A[base + (int64_t)(t + m * T) * a.NX + g * WT];
 
-  if (a.do_fwd) fft_line<N, P, 1, false>(v, t, a.tw, lline, LINE);
+  if (a.do_fwd) fft_line<N, P, G, false, GR, 3>(v, t, a.tw, lline, LINE);
   if (a.do_prop) {
     const int comp = (int)(slab % a.ncomp);
     if (a.pkind == PROP_SEP64) {
-      const double2 px = a.px[comp * a.NX + c];
+      double2 px[G];
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[0][m] = prop_sep_c64(v[0][m], px, a.py[comp * N + t + m * T], a.scale);
+      for (int g = 0; g < G; ++g) px[g] = a.px[comp * a.NX + c0 + g * WT];
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const double2 py = a.py[comp * N + t + m * T];
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[g][m] = prop_sep_c64(v[g][m], px[g], py, a.scale);
+      }
     } else {
-      const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c;
+      const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c0;
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[0][m] = prop_c64(v[0][m], pt[(int64_t)(t + m * T) * a.NX], a.scale);
+      for (int m = 0; m < P; ++m)
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[g][m] = prop_c64(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * WT], a.scale);
     }
   }
-  if (a.do_inv) fft_line<N, P, 1, true>(v, t, a.tw, lline, LINE);
+  if (a.do_inv) fft_line<N, P, G, true, GR, 3>(v, t, a.tw, lline, LINE);
 
+  if (a.out_scale != 1.f) {
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
-    float2 r = v[0][m];
-    r.x *= a.out_scale;
-    r.y *= a.out_scale;
-    a.A[base + (int64_t)(t + m * T) * a.NX] = r;
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].x * a.out_scale, v[g][m].y * a.out_scale);
   }
+#pragma unroll
+  for (int m = 0; m < P; ++m)
+#pragma unroll
+    for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
 }
 
 }  // namespace nlse

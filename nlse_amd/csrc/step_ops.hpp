@@ -42,16 +42,131 @@ struct RabiParams {
   float sn;   // sin(omega dz)
 };
 
+// sin and cos of h, |h| <= pi/4: Taylor series (coefficients 1/n!)
+// truncated where the remainder is below 2^-28 relative (sin to x^11,
+// cos to x^12 would be the next terms: 0.785^11/11! = 1.8e-9).
+__device__ __forceinline__ void sincos_small(float h, float& s, float& c) {
+  const float x2 = h * h;
+  float ps = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);  // 1/9!, -1/7!
+  ps = fmaf(x2, ps, 8.33333333e-3f);                      // 1/5!
+  ps = fmaf(x2, ps, -1.66666667e-1f);                     // -1/3!
+  s = fmaf(h * x2, ps, h);
+  float pc = fmaf(x2, -2.75573192e-7f, 2.48015873e-5f);   // -1/10!, 1/8!
+  pc = fmaf(x2, pc, -1.38888889e-3f);                     // -1/6!
+  pc = fmaf(x2, pc, 4.16666667e-2f);                      // 1/4!
+  pc = fmaf(x2, pc, -0.5f);                               // -1/2!
+  c = fmaf(x2, pc, 1.0f);
+}
+
+// expm1(x), |x| <= 1/8: Taylor series to x^7 (remainder 0.125^8/8! < 2e-12).
+__device__ __forceinline__ float expm1_small(float x) {
+  float p = fmaf(x, 1.98412698e-4f, 1.38888889e-3f);  // 1/7!, 1/6!
+  p = fmaf(x, p, 8.33333333e-3f);                      // 1/5!
+  p = fmaf(x, p, 4.16666667e-2f);                      // 1/4!
+  p = fmaf(x, p, 1.66666667e-1f);                      // 1/3!
+  p = fmaf(x, p, 0.5f);                                // 1/2!
+  return fmaf(x * x, p, x);
+}
+
+// 1/d to ~1 ulp: hardware reciprocal and one Newton step; exact for d = 1.
+__device__ __forceinline__ float recip(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(r, fmaf(-d, r, 1.0f), r);
+}
+
+// Library evaluation for arguments outside the polynomial range (kept out
+// of line: it is rarely taken and large).
+__device__ __noinline__ void sincos_expm1_lib(float h, float a, float* s, float* c, float* em) {
+  *em = expm1f(-a);
+  sincosf(h, s, c);
+}
+
 __device__ __forceinline__ float2 apply_phase_loss(float2 A, float a, float theta) {
-  const float em = expm1f(-a);
-  float s2, c2;
-  sincosf(0.5f * theta, &s2, &c2);
+  const float h = 0.5f * theta;
+  float em, s2, c2;
+  // wave-uniform choice: the polynomial range covers the physical steps
+  // (|theta| <= pi/2 and |a| <= 1/8 per step); otherwise the library calls
+  if (__builtin_amdgcn_read_exec() == __builtin_amdgcn_ballot_w64(fabsf(h) <= 0.78539816f && fabsf(a) <= 0.125f)) {
+    em = expm1_small(-a);
+    sincos_small(h, s2, c2);
+  } else {
+    sincos_expm1_lib(h, a, &s2, &c2, &em);
+  }
   const float cosm1 = -2.0f * s2 * s2;
   const float dr = fmaf(em, 1.0f + cosm1, cosm1);
   const float di = (1.0f + em) * (2.0f * s2 * c2);
   const float cx = fmaf(A.x, dr, -A.y * di);
   const float cy = fmaf(A.y, dr, A.x * di);
   return make_float2(A.x + cx, A.y + cy);
+}
+
+// Apply exp(-a + i theta) to P points with one wave-uniform choice of the
+// evaluation path for all of them.
+template <int P>
+__device__ __forceinline__ void apply_phase_loss_block(float2* A, const float* a, const float* theta) {
+  bool small = true;
+#pragma unroll
+  for (int m = 0; m < P; ++m) small = small && fabsf(0.5f * theta[m]) <= 0.78539816f && fabsf(a[m]) <= 0.125f;
+  if (__builtin_amdgcn_read_exec() == __builtin_amdgcn_ballot_w64(small)) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      float s2, c2;
+      const float em = expm1_small(-a[m]);
+      sincos_small(0.5f * theta[m], s2, c2);
+      const float cosm1 = -2.0f * s2 * s2;
+      const float dr = fmaf(em, 1.0f + cosm1, cosm1);
+      const float di = (1.0f + em) * (2.0f * s2 * c2);
+      const float cx = fmaf(A[m].x, dr, -A[m].y * di);
+      const float cy = fmaf(A[m].y, dr, A[m].x * di);
+      A[m] = make_float2(A[m].x + cx, A[m].y + cy);
+    }
+  } else {
+    for (int m = 0; m < P; ++m) {
+      float s2, c2, em;
+      sincos_expm1_lib(0.5f * theta[m], a[m], &s2, &c2, &em);
+      const float cosm1 = -2.0f * s2 * s2;
+      const float dr = fmaf(em, 1.0f + cosm1, cosm1);
+      const float di = (1.0f + em) * (2.0f * s2 * c2);
+      const float cx = fmaf(A[m].x, dr, -A[m].y * di);
+      const float cy = fmaf(A[m].y, dr, A[m].x * di);
+      A[m] = make_float2(A[m].x + cx, A[m].y + cy);
+    }
+  }
+}
+
+// nl_prop on P points (see nl1), potential values given per point.
+template <int P>
+__device__ __forceinline__ void nl1_block(float2* A, const struct NLParams1& p, bool loss_sat, const float* vr,
+                                          const float* vi) {
+  float a[P], th[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const float I = A[m].x * A[m].x + A[m].y * A[m].y;
+    const float sat = recip(fmaf(I, p.inv_isat, 1.0f));
+    th[m] = fmaf(p.gdz * I, sat, p.vdz * vr[m]);
+    a[m] = fmaf(p.vdz, vi[m], loss_sat ? p.adz * sat : p.adz);
+  }
+  apply_phase_loss_block<P>(A, a, th);
+}
+
+// nl_prop_c for both components of P points from one intensity snapshot.
+template <int P>
+__device__ __forceinline__ void nl2_block(float2* A1, float2* A2, const struct NLParams2& p1,
+                                          const struct NLParams2& p2, const float* vr, const float* vi) {
+  float a1[P], t1[P], a2[P], t2[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const float I1 = A1[m].x * A1[m].x + A1[m].y * A1[m].y;
+    const float I2 = A2[m].x * A2[m].x + A2[m].y * A2[m].y;
+    const float s1 = recip(fmaf(I2, p1.inv_is_o, fmaf(I1, p1.inv_is_s, 1.0f)));
+    const float s2 = recip(fmaf(I1, p2.inv_is_o, fmaf(I2, p2.inv_is_s, 1.0f)));
+    t1[m] = fmaf(p1.gs_dz * I1, s1, fmaf(p1.gc_dz * I2, s1, p1.vdz * vr[m]));
+    a1[m] = fmaf(p1.vdz, vi[m], p1.adz * s1);
+    t2[m] = fmaf(p2.gs_dz * I2, s2, fmaf(p2.gc_dz * I1, s2, p2.vdz * vr[m]));
+    a2[m] = fmaf(p2.vdz, vi[m], p2.adz * s2);
+  }
+  apply_phase_loss_block<P>(A1, a1, t1);
+  apply_phase_loss_block<P>(A2, a2, t2);
 }
 
 // V kinds
@@ -73,7 +188,7 @@ __device__ __forceinline__ void load_v(const void* V, int vkind, int64_t idx, fl
 // A complex V' = vr + i vi contributes i*V' = -vi + i vr.
 __device__ __forceinline__ float2 nl1(float2 A, const NLParams1& p, bool loss_sat, float vr, float vi) {
   const float I = A.x * A.x + A.y * A.y;
-  const float sat = 1.0f / (1.0f + I * p.inv_isat);
+  const float sat = recip(fmaf(I, p.inv_isat, 1.0f));
   const float theta = fmaf(p.gdz * I, sat, p.vdz * vr);
   const float a = fmaf(p.vdz, vi, loss_sat ? p.adz * sat : p.adz);
   return apply_phase_loss(A, a, theta);
@@ -81,7 +196,7 @@ __device__ __forceinline__ float2 nl1(float2 A, const NLParams1& p, bool loss_sa
 
 // nl_prop_c for one component from the shared intensity snapshot.
 __device__ __forceinline__ float2 nl2(float2 A, float Is, float Io, const NLParams2& p, float vr, float vi) {
-  const float sat = 1.0f / (1.0f + Is * p.inv_is_s + Io * p.inv_is_o);
+  const float sat = recip(fmaf(Io, p.inv_is_o, fmaf(Is, p.inv_is_s, 1.0f)));
   const float theta = fmaf(p.gs_dz * Is, sat, fmaf(p.gc_dz * Io, sat, p.vdz * vr));
   const float a = fmaf(p.vdz, vi, p.adz * sat);
   return apply_phase_loss(A, a, theta);

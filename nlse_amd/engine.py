@@ -131,6 +131,12 @@ class Plan2D:
         _lib.check(_lib.load().nlse_run(self._h, A.data_ptr(), int(nsteps), int(precision), int(step0),
                                         D.stream_ptr()), "run")
 
+    def run_pass(self, A: torch.Tensor, which: int, count: int, precision: int = 0) -> None:
+        """Launch one sweep type `count` times (0: row sweep, 1: column
+        sweep) in place on A -- timing only, the data is overwritten."""
+        _lib.check(_lib.load().nlse_run_pass(self._h, A.data_ptr(), int(which), int(count), int(precision),
+                                             D.stream_ptr()), "run_pass")
+
     def config(self) -> tuple[int, int]:
         a, b = C.c_int(), C.c_int()
         _lib.check(_lib.load().nlse_plan_config(self._h, C.byref(a), C.byref(b)), "plan_config")

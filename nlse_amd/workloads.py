@@ -1,0 +1,71 @@
+"""Synthetic inputs of the benchmark configurations (frozen recipes).
+
+Each builder takes the solver class (``nlse_amd.NLSE`` or an oracle class)
+so the engine and the CPU oracle are set up from the same numbers.
+
+  config0  2-D NLSE, README example form: 512^2, alpha=20, P=1.05 W,
+           window 4 w (w = 2.23 mm), n2=-1.6e-9, Isat=1e5, L=10 mm,
+           dz=1e-5 (1001 steps by the reference loop rule), Gaussian beam
+           times (1 + 0.01 complex Gaussian noise, seed 0), potential
+           V = -1e-4 exp(-r^2/(70 um)^2).
+  config1  the same physics at 2048^2, dz=5e-6 (2001 steps), nonlocal
+           length 20 um (optional), Gaussian beam plus a random-phase
+           speckle low-passed to 1/8 of the band (seed 1).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+WAIST = 2.23e-3
+WINDOW = 4 * WAIST
+L = 10e-3
+
+
+def _common(cls, N, **kw):
+    args = dict(alpha=20, power=1.05, window=WINDOW, n2=-1.6e-9, V=None, L=L, NX=N, NY=N, Isat=1e5, wvl=780e-9)
+    args.update(kw)
+    return cls(**args)
+
+
+def _potential(sim):
+    return (-1e-4 * np.exp(-(sim.XX**2 + sim.YY**2) / (70e-6) ** 2)).astype(np.float32)
+
+
+def config0(cls, N: int = 512, **kw):
+    sim = _common(cls, N, **kw)
+    sim.delta_z = 1e-5
+    sim.V = _potential(sim)
+    return sim
+
+
+def gaussian(sim):
+    return np.exp(-(sim.XX.astype(np.float64) ** 2 + sim.YY.astype(np.float64) ** 2) / WAIST**2)
+
+
+def config0_field(sim, seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    G = gaussian(sim)
+    nr = rng.standard_normal(G.shape)
+    ni = rng.standard_normal(G.shape)
+    return (G * (1 + 0.01 * (nr + 1j * ni))).astype(np.complex64)
+
+
+def config1(cls, N: int = 2048, nonlocal_length: float = 20e-6, **kw):
+    sim = _common(cls, N, nl_length=nonlocal_length, **kw)
+    sim.delta_z = 5e-6
+    sim.V = _potential(sim)
+    return sim
+
+
+def config1_field(sim, seed: int = 1) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    ny, nx = sim.XX.shape
+    phi = rng.uniform(0.0, 2 * np.pi, (ny, nx))
+    kx = np.fft.fftfreq(nx)[None, :]
+    ky = np.fft.fftfreq(ny)[:, None]
+    # |k| <= Kmax/8 on both axes (Kmax = pi/dx <-> |fftfreq| = 1/2)
+    mask = (np.abs(kx) <= 0.5 / 8) & (np.abs(ky) <= 0.5 / 8)
+    S = np.fft.ifft2(np.fft.fft2(np.exp(1j * phi)) * mask)
+    S /= np.sqrt(np.mean(np.abs(S) ** 2))
+    return (gaussian(sim) * (1 + S)).astype(np.complex64)

@@ -87,6 +87,7 @@ class NLSE:
         loss_saturates: bool = False,
         workers: int | None = None,
         fft: str = "ideal",
+        threads: int = 1,
     ) -> None:
         if mode not in ("c64", "c128"):
             raise ValueError("mode must be 'c64' or 'c128'")
@@ -94,6 +95,7 @@ class NLSE:
             raise ValueError("fft must be 'ideal' or 'pocketfft32'")
         self.mode = mode
         self.fft_kind = fft
+        self.threads = max(1, int(threads))
         self.loss_saturates = loss_saturates
         self.workers = workers or _workers()
         # nlse.py:90-114
@@ -221,7 +223,29 @@ class NLSE:
                 np.complex128 if np.iscomplexobj(V) else np.float64)
         return self.k / 2 * V
 
+    def _rows_parallel(self, fn, A, A_sq, V) -> None:
+        """Apply an elementwise operator over chunks of rows on a thread pool
+        (numpy releases the GIL), like numba's prange over the points."""
+        ny = A.shape[-2] if A.ndim >= 2 else 1
+        if self.threads <= 1 or A.ndim < 2 or ny < 2 * self.threads:
+            fn(A, A_sq, V)
+            return
+        from concurrent.futures import ThreadPoolExecutor
+
+        bounds = np.linspace(0, ny, self.threads + 1).astype(int)
+
+        def part(i):
+            sl = (Ellipsis, slice(bounds[i], bounds[i + 1]), slice(None))
+            Vs = None if V is None or np.ndim(V) < 2 else V[sl]
+            fn(A[sl], A_sq[sl], V if Vs is None and V is not None and np.ndim(V) < 2 else Vs)
+
+        with ThreadPoolExecutor(self.threads) as ex:
+            list(ex.map(part, range(self.threads)))
+
     def _nl(self, A, A_sq, V, dz) -> None:
+        self._rows_parallel(lambda a, s, v: self._nl_serial(a, s, v, dz), A, A_sq, V)
+
+    def _nl_serial(self, A, A_sq, V, dz) -> None:
         g = self.k / 2 * self.n2 * c * epsilon_0
         Isat = 2 * self.I_sat / (epsilon_0 * c)
         if V is None:
@@ -232,7 +256,7 @@ class NLSE:
                             self.loss_saturates)
 
     def _intensity(self, A, A_sq):
-        kernels.square_mod(A, A_sq)
+        self._rows_parallel(lambda a, s, v: kernels.square_mod(a, s), A, A_sq, None)
         if self.nl_length > 0:
             A_sq[...] = self._convolution(A_sq)
 
