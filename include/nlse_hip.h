@@ -119,15 +119,16 @@ int nlse_cavity_drive(void* A_dev, int64_t n_items, int64_t item_size, int64_t i
 int nlse_sumsq(const void* A_dev, int64_t n_items, int64_t item_size, double* out_dev, void* stream);
 
 /* ------------------------------------------------------ fused engine
- * A plan runs whole split-step loops (nlse.py:566-588) on one field shape:
- * 2-D fields (items, [2,] NY, NX).  Its tables are set once; nlse_run
- * then performs `nsteps` steps from real space to real space, two HBM
- * sweeps per step. */
+ * A plan runs whole split-step loops (nlse.py:566-588) on one field shape.
+ * Its tables are set once; nlse_run then performs `nsteps` steps from real
+ * space to real space: 2-D fields take two HBM sweeps per step, 3-D fields
+ * four, 1-D fields run all steps of a line in one launch. */
 typedef struct nlse_plan nlse_plan;
 
 typedef struct nlse_plan_desc {
-  int ndim;          /* 2 */
-  int64_t ny, nx;    /* transform sizes */
+  int ndim;          /* 1: (items, [2,] NX); 2: (items, [2,] NY, NX); 3: (items, NY, NX, NT) */
+  int64_t ny, nx;    /* transform sizes (ny ignored for ndim 1) */
+  int64_t nt;        /* third (fastest) axis of ndim 3 */
   int64_t items;     /* leading (broadcast) items */
   int ncomp;         /* 1, or 2 for CNLSE / DDGPE layouts (..., 2, NY, NX) */
   int loss_saturates;
@@ -153,6 +154,16 @@ int nlse_plan_destroy(nlse_plan* plan);
 /* Separable propagator: px [ncomp][nx], py [ncomp][ny] complex128 with
  * P(ky,kx) = complex64(px*py), the product taken in FP64. */
 int nlse_plan_set_propagator_sep(nlse_plan* plan, const double* px, const double* py);
+/* 1-D plans: complex128 propagator [ncomp][nx] (host memory), multiplied in
+ * FP64 like numpy's complex64*complex128 (nlse_1d.py:98-105). */
+int nlse_plan_set_propagator_c128(nlse_plan* plan, const double* prop);
+/* 3-D plans: the separable 2-D factors (nlse_plan_set_propagator_sep) plus
+ * the complex128 temporal factor pt[nt] = exp(-i D0/2 omega^2)
+ * (nlse_3d.py:105-114; no dz factor in the reference). */
+int nlse_plan_set_propagator_t(nlse_plan* plan, const double* pt);
+/* 3-D plans: a dense complex128 propagator [ny][nx][nt] in device memory
+ * owned by the caller (e.g. a propagator assigned by the user). */
+int nlse_plan_set_propagator_3d_c128(nlse_plan* plan, const void* prop_dev);
 /* Dense complex64 propagator [ncomp][ny][nx] in device memory owned by the
  * caller (kept alive by the caller while the plan uses it). */
 int nlse_plan_set_propagator_c64(nlse_plan* plan, const void* prop_dev);

@@ -31,6 +31,7 @@ class PlanDesc(C.Structure):
         ("ndim", c_int),
         ("ny", c_i64),
         ("nx", c_i64),
+        ("nt", c_i64),
         ("items", c_i64),
         ("ncomp", c_int),
         ("loss_saturates", c_int),
@@ -76,6 +77,9 @@ SIGNATURES = {
     "nlse_plan_destroy": (c_int, [c_void_p]),
     "nlse_plan_set_propagator_sep": (c_int, [c_void_p, c_void_p, c_void_p]),
     "nlse_plan_set_propagator_c64": (c_int, [c_void_p, c_void_p]),
+    "nlse_plan_set_propagator_c128": (c_int, [c_void_p, c_void_p]),
+    "nlse_plan_set_propagator_t": (c_int, [c_void_p, c_void_p]),
+    "nlse_plan_set_propagator_3d_c128": (c_int, [c_void_p, c_void_p]),
     "nlse_plan_set_nl1": (c_int, [c_void_p, c_int, c_void_p]),
     "nlse_plan_set_nl2": (c_int, [c_void_p, c_int, c_void_p]),
     "nlse_plan_set_rabi": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -1,0 +1,124 @@
+"""CNLSE on the MI355X engine (reference NLSE/cnlse.py:19-423).
+
+Field layout (..., 2, NY, NX).  The row sweep holds both components of a
+row in each thread, so both nonlinear steps use one intensity snapshot and
+the Rabi rotation is applied in registers (cnlse.py:374-423)."""
+
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+import torch
+from scipy.constants import c, epsilon_0
+
+from . import _device as D
+from .engine import NL2_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, inv_or_zero
+from .nlse import NLSE, bcast_items
+from .utils import __BACKEND__
+
+
+class CNLSE(NLSE):
+    """A class to solve the coupled NLSE."""
+
+    def __init__(self, alpha: float, power: float, window: float, n2: float, n12: float,
+                 V: Union[np.ndarray, None], L: float, NX: int = 1024, NY: int = 1024, Isat: float = np.inf,
+                 nl_length: float = 0, wvl: float = 780e-9, omega: float = None,
+                 backend: str = __BACKEND__) -> None:
+        super().__init__(alpha=alpha, power=power, window=window, n2=n2, V=V, L=L, NX=NX, NY=NY, Isat=Isat,
+                         nl_length=nl_length, wvl=wvl, backend=backend)
+        self.I_sat2 = self.I_sat
+        self.n12 = n12
+        self.n22 = self.n2
+        self.omega = omega
+        self.alpha2 = self.alpha
+        self.k2 = self.k
+        self.power2 = self.power
+        self.propagator1 = None
+        self.propagator2 = None
+
+    def _ncomp(self) -> int:
+        return 2
+
+    def _comp_axis_len(self) -> int:
+        return len(self._last_axes)
+
+    def _prepare_output_array(self, E, normalize: bool):
+        """cnlse.py:99-148: each component normalised to its power."""
+        Ed = self._field_to_device(E)
+        A = Ed.clone()
+        A_sq = torch.zeros(Ed.shape, dtype=torch.float32, device=Ed.device)
+        if normalize:
+            lead = self._lead(Ed)  # (..., 2)
+            plead = lead[:-1]
+            s = D.to_host(__import__("nlse_amd.kernels_hip", fromlist=["sumsq"]).sumsq(Ed, len(self._last_axes)))
+            integral = np.asarray(s).reshape(-1) * self._cell() * c * epsilon_0 / 2
+            p1 = bcast_items(self.power, plead)
+            p2 = bcast_items(self.power2, plead)
+            target = np.stack([p1, p2], axis=-1).reshape(-1)
+            scale = np.sqrt(target / integral)
+            isz = int(np.prod(Ed.shape[len(lead):]))
+            self._scale_items(A, scale, int(np.prod(lead)), isz, isz)
+        return A, A_sq
+
+    def _send_arrays_to_gpu(self) -> None:
+        super()._send_arrays_to_gpu()
+        for name in ("n22", "n12"):
+            if isinstance(getattr(self, name), np.ndarray):
+                setattr(self, name, D.to_device(getattr(self, name)))
+
+    def _retrieve_arrays_from_gpu(self) -> None:
+        super()._retrieve_arrays_from_gpu()
+        for name in ("n22", "n12"):
+            if isinstance(getattr(self, name), torch.Tensor):
+                setattr(self, name, D.to_host(getattr(self, name)))
+
+    def _build_propagator(self) -> np.ndarray:
+        """cnlse.py:179-190: one complex64 propagator per component (k, k2)."""
+        p1 = np.exp(-1j * 0.5 * (self.Kxx**2 + self.Kyy**2) / self.k * self.delta_z).astype(np.complex64)
+        p2 = np.exp(-1j * 0.5 * (self.Kxx**2 + self.Kyy**2) / self.k2 * self.delta_z).astype(np.complex64)
+        return np.array([p1, p2])
+
+    def _separable_factors(self):
+        px = np.stack([np.exp(-1j * 0.5 * self.Kx**2 / k * self.delta_z) for k in (self.k, self.k2)])
+        py = np.stack([np.exp(-1j * 0.5 * self.Ky**2 / k * self.delta_z) for k in (self.k, self.k2)])
+        return px, py
+
+    def _take_components(self, A):
+        """cnlse.py:192-202."""
+        return A[..., 0, :, :], A[..., 1, :, :]
+
+    def _nl2_records(self, dz: float, lead: tuple) -> np.ndarray:
+        """Both nl_prop[_without_V]_c calls of one application (cnlse.py:251-298)."""
+        b = lambda x: bcast_items(x, lead)  # noqa: E731
+        isat1 = 2 * b(self.I_sat) / (epsilon_0 * c)
+        isat2 = 2 * b(self.I_sat2) / (epsilon_0 * c)
+        rec = np.zeros((b(0.0).size, 2), NL2_DTYPE)
+        rec["adz"][:, 0] = dz * (b(self.alpha) / 2)
+        rec["gs_dz"][:, 0] = dz * (self.k / 2 * b(self.n2) * c * epsilon_0)
+        rec["gc_dz"][:, 0] = dz * (self.k / 2 * b(self.n12) * c * epsilon_0)
+        rec["inv_is_s"][:, 0] = inv_or_zero(isat1)
+        rec["inv_is_o"][:, 0] = inv_or_zero(isat2)
+        rec["vdz"][:, 0] = dz * (self.k / 2)
+        rec["adz"][:, 1] = dz * (b(self.alpha2) / 2)
+        rec["gs_dz"][:, 1] = dz * (self.k2 / 2 * b(self.n22) * c * epsilon_0)
+        rec["gc_dz"][:, 1] = dz * (self.k2 / 2 * b(self.n12) * c * epsilon_0)
+        rec["inv_is_s"][:, 1] = inv_or_zero(isat2)
+        rec["inv_is_o"][:, 1] = inv_or_zero(isat1)
+        rec["vdz"][:, 1] = dz * (self.k2 / 2)
+        return rec.reshape(-1)
+
+    def _nl_slots(self, precision: str, lead: tuple, V=None) -> dict:
+        dz = self.delta_z
+        if precision == "double":
+            half = self._nl2_records(dz / 2, lead)
+            return {SLOT_HALF_PRE: half, SLOT_HALF_POST: half}
+        return {SLOT_FULL: self._nl2_records(dz, lead)}
+
+    def _rabi_params(self, lead: tuple):
+        """rabi_coupling(A1, A2, dz, omega/2) of the single-precision step only
+        (cnlse.py:422-423, quirk Q14)."""
+        if self.omega is None:
+            return None
+        w = bcast_items(self.omega, lead) / 2 * self.delta_z
+        return -2.0 * np.sin(w / 2) ** 2, np.sin(w)

@@ -1,0 +1,45 @@
+"""CNLSE_1d on the MI355X engine (reference NLSE/cnlse_1d.py:13-131)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _device as D
+from .cnlse import CNLSE
+from .utils import __BACKEND__
+
+
+class CNLSE_1d(CNLSE):
+    """A class to solve the 1D coupled NLSE; layout (..., 2, NX)."""
+
+    def __init__(self, alpha: float, power: float, window: float, n2: float, n12: float, V: np.ndarray, L: float,
+                 NX: int = 1024, Isat: float = np.inf, nl_length: float = 0, wvl: float = 780e-9,
+                 omega: float = None, backend: str = __BACKEND__) -> None:
+        super().__init__(alpha=alpha, power=power, window=window, n2=n2, n12=n12, V=V, L=L, NX=NX, Isat=Isat,
+                         nl_length=nl_length, wvl=wvl, omega=omega, backend=backend)
+        self._last_axes = (-1,)
+        self.nl_profile = self.nl_profile[0]
+        self.nl_profile /= self.nl_profile.sum()
+
+    def _cell(self) -> float:
+        return self.delta_X**2  # cnlse_1d.py:97
+
+    def _take_components(self, A):
+        return A[..., 0, :], A[..., 1, :]
+
+    def _build_propagator(self) -> np.ndarray:
+        """cnlse_1d.py:119-131: complex128."""
+        p1 = np.exp(-1j * 0.5 * (self.Kx**2) / self.k * self.delta_z)
+        p2 = np.exp(-1j * 0.5 * (self.Kx**2) / self.k2 * self.delta_z)
+        return np.array([p1, p2])
+
+    def _separable_factors(self):
+        return None
+
+    def _engine_geometry(self, A) -> dict:
+        lead = self._param_lead(A)
+        return dict(ndim=1, ny=1, nx=self.NX, nt=1, items=int(np.prod(lead)) if lead else 1, ncomp=2)
+
+    def _configure_tables(self, eng, V, propagator) -> None:
+        eng.set_propagator_c128(np.asarray(D.to_host(propagator), dtype=np.complex128))
+        eng.set_potential(None if V is None else D.to_device(V))

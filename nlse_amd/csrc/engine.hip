@@ -140,6 +140,21 @@ static int row_launch(int G, const RowArgs& a, hipStream_t s) {
   return launch_status("k_row");
 }
 
+template <int N>
+static int line1d_launch(int G, const Line1DArgs& a, hipStream_t s) {
+  using C = LineCfg<N>;
+  if (G == 1) {
+    constexpr int L = C::LPB1;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    hipLaunchKernelGGL((k_line1d<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  } else {
+    constexpr int L = C::LPB2;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    hipLaunchKernelGGL((k_line1d<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  }
+  return launch_status("k_line1d");
+}
+
 template <int N, int W>
 static bool col_launch_w(const ColArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
@@ -197,6 +212,7 @@ static int col_launch(int W, const ColArgs& a, hipStream_t s) {
 struct LineOps {
   int (*twiddles)(const float4**);
   int (*row)(int, const RowArgs&, hipStream_t);
+  int (*line1d)(int, const Line1DArgs&, hipStream_t);
   int (*col)(int, const ColArgs&, hipStream_t);
   int (*colw)(int64_t, int);
   int lpb1, lpb2, t;
@@ -205,7 +221,7 @@ struct LineOps {
 template <int N>
 static LineOps make_ops() {
   using C = LineCfg<N>;
-  return LineOps{&get_twiddles<N, C::P>, &row_launch<N>, &col_launch<N>, &col_width_for<N>,
+  return LineOps{&get_twiddles<N, C::P>, &row_launch<N>, &line1d_launch<N>, &col_launch<N>, &col_width_for<N>,
                  C::LPB1, C::LPB2, C::T};
 }
 
@@ -593,11 +609,17 @@ int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* str
 // ------------------------------------------------------- fused plan
 struct nlse_plan {
   nlse_plan_desc d;
-  const LineOps* rops = nullptr;  // rows (NX)
+  const LineOps* rops = nullptr;  // rows (NX) / x columns in 3-D
   const LineOps* cops = nullptr;  // columns (NY)
+  const LineOps* tops = nullptr;  // t rows (NT, 3-D)
   const float4* twx = nullptr;
   const float4* twy = nullptr;
-  int colw = 1;
+  const float4* twt = nullptr;
+  int colw = 1;                   // y-column tile
+  int colw_x = 1;                 // x-column tile (3-D)
+  double2* prop128 = nullptr;     // 1-D complex128 propagator [ncomp][nx]
+  double2* pt = nullptr;          // 3-D temporal factor [nt]
+  const double2* ptab128 = nullptr;  // 3-D dense complex128 table (caller-owned)
   // propagator
   int pkind = -1;
   double2* px = nullptr;
@@ -620,25 +642,32 @@ struct nlse_plan {
 int nlse_plan_create(const nlse_plan_desc* d, nlse_plan** out) {
   if (!out || !d) return fail(NLSE_ERR_ARG, "null argument");
   *out = nullptr;
-  if (d->ndim != 2) return fail(NLSE_ERR_ARG, "nlse_plan supports ndim=2 (got %d)", d->ndim);
+  if (d->ndim < 1 || d->ndim > 3) return fail(NLSE_ERR_ARG, "nlse_plan supports ndim 1, 2, 3 (got %d)", d->ndim);
   if (d->items < 1 || (d->ncomp != 1 && d->ncomp != 2))
     return fail(NLSE_ERR_ARG, "bad items/ncomp (%lld, %d)", (long long)d->items, d->ncomp);
+  if (d->ndim == 3 && d->ncomp != 1) return fail(NLSE_ERR_ARG, "3-D plans have one component");
   nlse_plan* p = new nlse_plan();
   p->d = *d;
   p->rops = line_ops(d->nx);
-  p->cops = line_ops(d->ny);
-  if (!p->rops || !p->cops) {
+  p->cops = d->ndim >= 2 ? line_ops(d->ny) : p->rops;
+  p->tops = d->ndim == 3 ? line_ops(d->nt) : p->rops;
+  if (!p->rops || !p->cops || !p->tops) {
     delete p;
-    return fail(NLSE_ERR_ARG, "grid %lldx%lld unsupported (powers of two in [8, 4096])", (long long)d->ny,
-                (long long)d->nx);
+    return fail(NLSE_ERR_ARG, "grid (%lld, %lld, %lld) unsupported (powers of two in [8, 4096])", (long long)d->ny,
+                (long long)d->nx, (long long)d->nt);
   }
   int rc = p->rops->twiddles(&p->twx);
   if (!rc) rc = p->cops->twiddles(&p->twy);
+  if (!rc) rc = p->tops->twiddles(&p->twt);
   if (rc) {
     delete p;
     return rc;
   }
-  p->colw = p->cops->colw(d->nx, d->col_width);
+  if (d->ndim == 2) p->colw = p->cops->colw(d->nx, d->col_width);
+  if (d->ndim == 3) {
+    p->colw = p->cops->colw(d->nx * d->nt, d->col_width);
+    p->colw_x = p->rops->colw(d->nt, d->col_width);
+  }
   *out = p;
   return NLSE_OK;
 }
@@ -648,12 +677,15 @@ int nlse_plan_destroy(nlse_plan* p) {
   (void)hipFree(p->px);
   (void)hipFree(p->py);
   for (void* q : p->nl) (void)hipFree(q);
+  (void)hipFree(p->prop128);
+  (void)hipFree(p->pt);
   (void)hipFree(p->rabi);
   delete p;
   return NLSE_OK;
 }
 
 int nlse_plan_config(const nlse_plan* p, int* row_lines, int* col_width) {
+  // 3-D plans report their y-column tile
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
   if (row_lines) *row_lines = p->d.ncomp == 1 ? p->rops->lpb1 : p->rops->lpb2;
   if (col_width) *col_width = p->colw;
@@ -668,16 +700,41 @@ static int upload(void** dst, const void* src, size_t bytes) {
 
 int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* py) {
   if (!p || !px || !py) return fail(NLSE_ERR_ARG, "null argument");
+  if (p->d.ndim == 1) return fail(NLSE_ERR_ARG, "1-D plans take nlse_plan_set_propagator_c128");
   const int nc = p->d.ncomp;
   int rc = upload((void**)&p->px, px, sizeof(double2) * nc * p->d.nx);
   if (!rc) rc = upload((void**)&p->py, py, sizeof(double2) * nc * p->d.ny);
   if (rc) return rc;
-  p->pkind = PROP_SEP64;
+  p->pkind = p->d.ndim == 3 ? PROP_SEP3D : PROP_SEP64;
+  return NLSE_OK;
+}
+
+int nlse_plan_set_propagator_c128(nlse_plan* p, const double* prop) {
+  if (!p || !prop) return fail(NLSE_ERR_ARG, "null argument");
+  if (p->d.ndim != 1) return fail(NLSE_ERR_ARG, "complex128 line propagator is for 1-D plans");
+  int rc = upload((void**)&p->prop128, prop, sizeof(double2) * p->d.ncomp * p->d.nx);
+  if (rc) return rc;
+  p->pkind = PROP_C64;  // any non-negative kind marks the plan ready
+  return NLSE_OK;
+}
+
+int nlse_plan_set_propagator_t(nlse_plan* p, const double* pt) {
+  if (!p || !pt) return fail(NLSE_ERR_ARG, "null argument");
+  if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "temporal propagator factor is for 3-D plans");
+  return upload((void**)&p->pt, pt, sizeof(double2) * p->d.nt);
+}
+
+int nlse_plan_set_propagator_3d_c128(nlse_plan* p, const void* tab) {
+  if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
+  if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "dense complex128 propagators are for 3-D plans");
+  p->ptab128 = (const double2*)tab;
+  p->pkind = PROP_C128;
   return NLSE_OK;
 }
 
 int nlse_plan_set_propagator_c64(nlse_plan* p, const void* tab) {
   if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
+  if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "dense complex64 propagators are for 2-D plans");
   p->ptab = (const float2*)tab;
   p->pkind = PROP_C64;
   return NLSE_OK;
@@ -809,11 +866,97 @@ static RowArgs mid_row(const nlse_plan* p, const RowArgs& r, int precision, bool
   return a;
 }
 
+static int run1d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream_t s) {
+  Line1DArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.A = (float2*)A;
+  a.ngroups = p->d.items;
+  a.tw = p->twx;
+  a.prop = p->prop128;
+  a.scale = 1.0 / (double)p->d.nx;
+  a.nsteps = nsteps;
+  a.precision = precision;
+  a.loss_sat = p->d.loss_saturates;
+  a.nl_full = p->nl[NLSE_SLOT_FULL];
+  a.nl_pre = p->nl[NLSE_SLOT_HALF_PRE];
+  a.nl_post = p->nl[NLSE_SLOT_HALF_POST];
+  a.rabi = (precision == 0 && p->d.ncomp == 2) ? p->rabi : nullptr;
+  a.V = p->V;
+  a.vkind = p->vkind;
+  return p->rops->line1d(p->d.ncomp, a, s);
+}
+
+// 3-D field (items, NY, NX, NT), t fastest.  Between sweeps the field is
+// t- and x-spectral; one step is four sweeps:
+//   y columns: F_y, P/(NX NY NT), I_y   x columns: I_x
+//   t rows:    I_t, N, F_t               x columns: F_x
+static int run3d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream_t s) {
+  const int64_t ny = p->d.ny, nx = p->d.nx, nt = p->d.nt, items = p->d.items;
+  RowArgs r = row_defaults();
+  r.A = (float2*)A;
+  r.rows_per_item = ny * nx;
+  r.ngroups = items * ny * nx;
+  r.tw = p->twt;
+  r.loss_sat = p->d.loss_saturates;
+  r.V = p->V;
+  r.vkind = p->vkind;
+  r.v_item_stride = p->v_item_stride;
+  ColArgs cx = col_defaults();
+  cx.A = (float2*)A;
+  cx.nslabs = items * ny;
+  cx.NX = (int)nt;
+  cx.tw = p->twx;
+  ColArgs cy = col_defaults();
+  cy.A = (float2*)A;
+  cy.nslabs = items;
+  cy.NX = (int)(nx * nt);
+  cy.tw = p->twy;
+  cy.do_fwd = cy.do_prop = cy.do_inv = 1;
+  cy.pkind = p->pkind;
+  cy.ptab128 = p->ptab128;
+  cy.px = p->px;
+  cy.py = p->py;
+  cy.pt = p->pt;
+  cy.nt = (int)nt;
+  cy.scale = (float)(1.0 / ((double)nx * (double)ny * (double)nt));
+  int rc;
+  {
+    RowArgs a = r;
+    a.forward_last = 1;
+    a.has_b = precision == 1;
+    a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
+    if ((rc = p->tops->row(1, a, s))) return rc;
+    ColArgs c = cx;
+    c.do_fwd = 1;
+    if ((rc = p->rops->col(p->colw_x, c, s))) return rc;
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    const bool last = (k == nsteps - 1);
+    if ((rc = p->cops->col(p->colw, cy, s))) return rc;
+    ColArgs ci = cx;
+    ci.do_inv = 1;
+    if ((rc = p->rops->col(p->colw_x, ci, s))) return rc;
+    RowArgs a = mid_row(p, r, precision, last);
+    if ((rc = p->tops->row(1, a, s))) return rc;
+    if (!last) {
+      ColArgs cf = cx;
+      cf.do_fwd = 1;
+      if ((rc = p->rops->col(p->colw_x, cf, s))) return rc;
+    }
+  }
+  return NLSE_OK;
+}
+
 int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, void* stream) {
   if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
   if (nsteps == 0) return NLSE_OK;
   int rc = check_ready(p, precision);
   if (rc) return rc;
+  if (p->d.ndim == 1) return run1d(p, A, nsteps, precision, (hipStream_t)stream);
+  if (p->d.ndim == 3) {
+    if (p->pkind == PROP_SEP3D && !p->pt) return fail(NLSE_ERR_STATE, "3-D temporal propagator factor not set");
+    return run3d(p, A, nsteps, precision, (hipStream_t)stream);
+  }
   if (p->drive && step0 + nsteps > (int64_t)p->ft0.size())
     return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
                 (long long)(step0 + nsteps - 1));
@@ -848,6 +991,7 @@ int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0
 int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision, void* stream) {
   if (!p || !A || count < 0 || (pass != 0 && pass != 1) || (precision != 0 && precision != 1))
     return fail(NLSE_ERR_ARG, "bad run_pass arguments");
+  if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "run_pass times the sweeps of 2-D plans");
   int rc = check_ready(p, precision);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;

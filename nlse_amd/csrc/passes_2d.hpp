@@ -18,7 +18,7 @@
 
 namespace nlse {
 
-enum : int { PROP_SEP64 = 0, PROP_C64 = 1 };
+enum : int { PROP_SEP64 = 0, PROP_C64 = 1, PROP_SEP3D = 2, PROP_C128 = 3 };
 
 struct RowArgs {
   float2* A;
@@ -161,6 +161,9 @@ struct ColArgs {
   const double2* px;   // [ncomp][NX]   separable factors exp(-i phi_x)
   const double2* py;   // [ncomp][NY]   exp(-i phi_y)
   const float2* ptab;  // [ncomp][NY][NX] dense complex64 propagator
+  const double2* pt;   // [NT] temporal factor of the 3-D propagator (PROP_SEP3D)
+  const double2* ptab128;  // [N][NX] dense complex128 propagator (PROP_C128)
+  int nt;              // columns = NX * NT for PROP_SEP3D
   float scale;         // 1/(NX NY) folded into the propagator
   float out_scale;     // applied to the stored result when no propagator
 };
@@ -213,6 +216,29 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep_c64(v[g][m], px[g], py, a.scale);
       }
+    } else if (a.pkind == PROP_SEP3D) {
+      // column c = x*NT + t of an (NY, NX, NT) field: complex64(px*py)
+      // times the complex128 temporal factor, applied in FP64
+      double2 px[G], pt[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int col = c0 + g * WT;
+        px[g] = a.px[col / a.nt];
+        pt[g] = a.pt[col % a.nt];
+      }
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const double2 py = a.py[t + m * T];
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[g][m] = prop_sep3d(v[g][m], px[g], py, pt[g], (double)a.scale);
+      }
+    } else if (a.pkind == PROP_C128) {
+      const double2* pt = a.ptab128 + c0;
+#pragma unroll
+      for (int m = 0; m < P; ++m)
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          v[g][m] = prop_c128(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * WT], (double)a.scale);
     } else {
       const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c0;
 #pragma unroll
@@ -233,6 +259,102 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   for (int m = 0; m < P; ++m)
 #pragma unroll
     for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
+}
+
+// ----------------------------------------------------------------- 1-D
+// Whole 1-D runs: every line is independent, so one launch performs all
+// `nsteps` steps of LPB lines (G = 1 or component pairs G = 2) in
+// registers and LDS: [N(dz/2)] F, * P/N (complex128 propagator, FP64
+// multiply), I, N(dz or dz/2).  16 B per point for the whole run.
+struct Line1DArgs {
+  float2* A;
+  int64_t ngroups;        // lines (G=1) or line pairs (G=2)
+  const float4* tw;
+  const double2* prop;    // [ncomp][N] complex128 propagator
+  double scale;           // 1/N
+  int64_t nsteps;
+  int precision;          // 0 single, 1 double
+  int loss_sat;
+  const void* nl_full;    // NLParams1[items] / NLParams2[items*2]
+  const void* nl_pre;
+  const void* nl_post;
+  const RabiParams* rabi; // [items] or null
+  const void* V;          // [N] raw potential
+  int vkind;
+};
+
+template <int N, int P, int LPB, int G>
+__global__ __launch_bounds__(LPB*(N / P)) void k_line1d(Line1DArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  __shared__ float2 lds[LPB * G * LINE];
+  const int li = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t gi = (int64_t)blockIdx.x * LPB + li;
+  const bool active = gi < a.ngroups;
+  const int64_t item = active ? gi : 0;
+  int64_t off[G];
+  off[0] = item * G * N;
+  if constexpr (G == 2) off[1] = off[0] + N;
+  float2* lline = lds + li * G * LINE;
+
+  float2 v[G][P];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[off[g] + t + m * T] : make_float2(0.f, 0.f);
+  float vr[P], vi[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    vr[m] = 0.f;
+    vi[m] = 0.f;
+    if (a.vkind != V_NONE) load_v(a.V, a.vkind, t + m * T, vr[m], vi[m]);
+  }
+  double2 pr[G][P];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) pr[g][m] = a.prop[g * N + t + m * T];
+
+  auto apply_nl = [&](const void* prm) {
+    if constexpr (G == 1) {
+      const NLParams1 p = reinterpret_cast<const NLParams1*>(prm)[item];
+      nl1_block<P>(v[0], p, a.loss_sat, vr, vi);
+    } else {
+      const NLParams2* pp = reinterpret_cast<const NLParams2*>(prm) + 2 * item;
+      const NLParams2 p1 = pp[0], p2 = pp[1];
+      nl2_block<P>(v[0], v[1], p1, p2, vr, vi);
+    }
+  };
+
+  for (int64_t s = 0; s < a.nsteps; ++s) {
+    if (a.precision == 1) apply_nl(a.nl_pre);
+    fft_line<N, P, G, false>(v, t, a.tw, lline, LINE);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = prop_c128(v[g][m], pr[g][m], a.scale);
+    fft_line<N, P, G, true>(v, t, a.tw, lline, LINE);
+    if (a.precision == 1) {
+      apply_nl(a.nl_post);
+    } else {
+      apply_nl(a.nl_full);
+      if constexpr (G == 2) {
+        if (a.rabi != nullptr) {
+          const RabiParams rb = a.rabi[item];
+#pragma unroll
+          for (int m = 0; m < P; ++m) rabi(v[0][m], v[1][m], rb);
+        }
+      }
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) a.A[off[g] + t + m * T] = v[g][m];
+  }
 }
 
 }  // namespace nlse

@@ -234,4 +234,16 @@ __device__ __forceinline__ float2 prop_c128(float2 A, double2 P, double scale) {
   return make_float2((float)fma(ax, px, -ay * py), (float)fma(ax, py, ay * px));
 }
 
+// 3-D propagator (nlse_3d.py:105-114): P = complex64(px*py) (the 2-D part,
+// cast to complex64 by the reference) times the complex128 temporal
+// factor; the product and A*P are evaluated in FP64 and rounded once.
+__device__ __forceinline__ float2 prop_sep3d(float2 A, double2 px, double2 py, double2 pt, double scale) {
+  const double p2r = (double)(float)fma(px.x, py.x, -px.y * py.y);
+  const double p2i = (double)(float)fma(px.x, py.y, px.y * py.x);
+  const double pr = fma(p2r, pt.x, -p2i * pt.y) * scale;
+  const double pi = fma(p2r, pt.y, p2i * pt.x) * scale;
+  const double ax = A.x, ay = A.y;
+  return make_float2((float)fma(ax, pr, -ay * pi), (float)fma(ax, pi, ay * pr));
+}
+
 }  // namespace nlse

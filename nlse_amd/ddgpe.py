@@ -1,0 +1,163 @@
+"""DDGPE on the MI355X engine (reference NLSE/ddgpe.py:14-565).
+
+Component 0 is the exciton field, component 1 the cavity field.  The
+cavity dispersion is not separable, so the propagator is a dense complex64
+table.  When the laser drive (DDGPE.laser_excitation) is the only callback
+of a run it is applied inside the fused row sweep (after the step, before
+the next transform); with other callbacks every step returns to the host and
+the drive runs as a device operator (nlse_cavity_drive)."""
+
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+from .cnlse import CNLSE
+from .engine import NL2_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, inv_or_zero
+from .nlse import NLSE, bcast_items
+from .utils import __BACKEND__
+
+
+def _drive(A: torch.Tensor, Fr, Ft, dz: float) -> None:
+    """A[..., 1, :, :] -= Fr * Ft * dz * 1j (ddgpe.py:155-156), one term."""
+    view = A[..., 1, :, :]
+    isz = view.shape[-1] * view.shape[-2]
+    items = view.numel() // isz
+    stride = A.shape[-1] * A.shape[-2] * 2
+    Fr_t = D.to_device(Fr, torch.complex64)
+    if Fr_t.numel() != isz:
+        raise ValueError("drive profile must have the transverse shape (NY, NX)")
+    ft = np.complex64(D.to_host(Ft) if not np.isscalar(Ft) else Ft)
+    _lib.check(_lib.load().nlse_cavity_drive(view.data_ptr(), items, isz, stride, Fr_t.data_ptr(), float(ft.real),
+                                             float(ft.imag), float(dz), D.stream_ptr()), "cavity_drive")
+
+
+class DDGPE(CNLSE):
+    """A class to solve the 2D driven dissipative Gross-Pitaevskii equation."""
+
+    def __init__(self, gamma: float, power: float, window: float, g: float, omega: float, T: float,
+                 omega_exc: float, omega_cav: float, detuning: float, k_z: float, V: np.ndarray = None,
+                 g12: float = 0, NX: int = 1024, NY: int = 1024, Isat: float = np.inf, nl_length: float = 0,
+                 backend: str = __BACKEND__) -> None:
+        super().__init__(alpha=gamma, power=power, window=window, n2=-g, n12=g12, V=V, L=T, NX=NX, NY=NY,
+                         Isat=Isat, nl_length=nl_length, wvl=1e-30, omega=omega, backend=backend)
+        self.g = self.n2
+        self.g12 = self.n12
+        self.g2 = 0
+        self.k_z = k_z
+        self.gamma = gamma
+        self.gamma2 = self.gamma
+        self.omega_exc = omega_exc
+        self.omega_cav = omega_cav
+        self.detuning = detuning
+        omega_lp = (omega_exc + omega_cav) / 2 - 0.5 * np.sqrt((omega_exc - omega_cav) ** 2 + (omega) ** 2)
+        self.omega_pump = omega_lp + detuning
+
+    @staticmethod
+    def laser_excitation(simu, A, t, i, F_pump_r, F_pump_t, F_probe_r, F_probe_t) -> None:
+        """Pump and probe drive of the cavity field (ddgpe.py:127-156)."""
+        _drive(A, F_pump_r, F_pump_t[i], simu.delta_z)
+        _drive(A, F_probe_r, F_probe_t[i], simu.delta_z)
+
+    _EXTRA = ("gamma", "g", "omega", "k_z", "omega_exc", "omega_cav", "detuning", "omega_pump")
+
+    def _send_arrays_to_gpu(self) -> None:
+        super()._send_arrays_to_gpu()
+        for name in self._EXTRA:
+            if isinstance(getattr(self, name), np.ndarray):
+                setattr(self, name, D.to_device(getattr(self, name)))
+
+    def _retrieve_arrays_from_gpu(self) -> None:
+        super()._retrieve_arrays_from_gpu()
+        for name in self._EXTRA:
+            if isinstance(getattr(self, name), torch.Tensor):
+                setattr(self, name, D.to_host(getattr(self, name)))
+
+    def _build_propagator(self) -> np.ndarray:
+        """ddgpe.py:204-223: flat exciton, polariton cavity dispersion."""
+        p1 = np.exp(-1j * (self.omega_exc * (1 + 0 * self.Kxx**2) - self.omega_pump) * self.delta_z).astype(
+            np.complex64)
+        p2 = np.exp(-1j * (self.omega_cav * np.sqrt(1 + (self.Kxx**2 + self.Kyy**2) / self.k_z**2)
+                           - self.omega_pump) * self.delta_z).astype(np.complex64)
+        return np.array([p1, p2])
+
+    def _separable_factors(self):
+        return None  # the cavity dispersion is not separable
+
+    def _prepare_output_array(self, E_in, normalize: bool):
+        """ddgpe.py:226-245: never normalises."""
+        E = self._field_to_device(E_in)
+        return E.clone(), torch.zeros(E.shape, dtype=torch.float32, device=E.device)
+
+    def _records(self, lead, comps) -> np.ndarray:
+        """comps: two tuples (dz, alpha, g_self, g_cross, vdz) for the exciton
+        and cavity applications; I_sat, I_sat2 pair with (I_self, I_other)
+        in the argument order of ddgpe.py (I_sat with the first intensity)."""
+        b = lambda x: bcast_items(x, lead)  # noqa: E731
+        n = b(0.0).size
+        rec = np.zeros((n, 2), NL2_DTYPE)
+        for j, (dz, alpha, gs, gc, vdz) in enumerate(comps):
+            rec["adz"][:, j] = dz * (b(alpha) / 2)
+            rec["gs_dz"][:, j] = dz * b(gs)
+            rec["gc_dz"][:, j] = dz * b(gc)
+            rec["inv_is_s"][:, j] = inv_or_zero(b(self.I_sat))
+            rec["inv_is_o"][:, j] = inv_or_zero(b(self.I_sat2))
+            rec["vdz"][:, j] = vdz
+        return rec.reshape(-1)
+
+    def _nl_slots(self, precision: str, lead: tuple, V=None) -> dict:
+        """ddgpe.py:278-457, including the inconsistent double mode (quirk
+        Q15): with V the pre-step uses dz and raw V, the post-step k/2 V for
+        the exciton; without V the cavity uses g (not g2) in double mode."""
+        dz, g, g12, g2 = self.delta_z, self.g, self.g12, self.g2
+        hasV = V is not None
+        vd = dz if hasV else 0.0
+        if precision == "double":
+            if hasV:
+                pre = self._records(lead, [(dz, self.gamma, g, g12, vd), (dz, self.gamma2, g2, g12, vd)])
+                post = self._records(lead, [(dz, self.gamma, g, g12, dz * self.k / 2),
+                                            (dz, self.gamma2, g2, g12, vd)])
+            else:
+                h = dz / 2
+                pre = self._records(lead, [(h, self.gamma, g, g12, 0.0), (h, self.gamma2, g, g12, 0.0)])
+                post = pre
+            return {SLOT_HALF_PRE: pre, SLOT_HALF_POST: post}
+        if hasV:
+            full = self._records(lead, [(dz, self.gamma, g, g12, vd), (dz, self.gamma2, g2, g12, vd)])
+        else:
+            full = self._records(lead, [(dz, self.alpha, g, g12, 0.0), (dz, self.gamma2, g2, g12, 0.0)])
+        return {SLOT_FULL: full}
+
+    # -- callbacks fused into the engine -------------------------------------
+    def _fuse_callbacks(self, callback, callback_args):
+        if (isinstance(callback, list) and len(callback) == 1 and callback[0] is DDGPE.laser_excitation
+                and callback_args is not None and len(callback_args) >= 1):
+            return None, (), tuple(callback_args[0])
+        return callback, callback_args, None
+
+    def _run_fused(self, A, V, nsteps: int, precision: str, fused) -> None:
+        eng = self._engine_for(A)
+        if fused is not None:
+            Fr0, Ft0, Fr1, Ft1 = fused
+            eng.set_drive(Fr0, Fr1, np.asarray(D.to_host(Ft0)), np.asarray(D.to_host(Ft1)), self.delta_z)
+        try:
+            self._run_chunk(A, V, nsteps, precision, 0)
+        finally:
+            eng.set_drive(None, None, None, None, 0.0)
+
+    def out_field(self, E_in, t: float, laser_excitation: Union[callable, None], plot: bool = False,
+                  precision: str = "single", verbose: bool = True, callback=None, callback_args=None):
+        """ddgpe.py:519-565: the drive becomes callback 0 (the caller's list is
+        mutated), the field is not normalised and None is returned (Q16).
+        The propagated field is kept in self.field."""
+        if laser_excitation is None:
+            callback.insert(0, self.laser_excitation)
+        else:
+            callback.insert(0, laser_excitation)
+        self.field = NLSE.out_field(self, E_in=E_in, z=t, plot=plot, precision=precision, verbose=verbose,
+                                    normalize=False, callback=callback, callback_args=callback_args)
+        return None

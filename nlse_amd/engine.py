@@ -23,14 +23,19 @@ def inv_or_zero(x):
         return np.where(np.isinf(x), 0.0, 1.0 / x)
 
 
-class Plan2D:
-    """Fused 2-D engine for fields (items, [2,] ny, nx) of complex64."""
+class Plan:
+    """Fused engine for complex64 fields (items, [2,] nx) (ndim 1),
+    (items, [2,] ny, nx) (ndim 2) or (items, ny, nx, nt) (ndim 3)."""
 
     def __init__(self, ny: int, nx: int, items: int, ncomp: int = 1, loss_saturates: bool = False,
-                 col_width: int = 0):
-        self.ny, self.nx, self.items, self.ncomp = int(ny), int(nx), int(items), int(ncomp)
-        desc = _lib.PlanDesc(2, self.ny, self.nx, self.items, self.ncomp, int(bool(loss_saturates)), 0,
-                             int(col_width))
+                 col_width: int = 0, ndim: int = 2, nt: int = 1):
+        self.ndim = int(ndim)
+        self.ny = int(ny) if self.ndim >= 2 else 1
+        self.nx, self.nt = int(nx), (int(nt) if self.ndim == 3 else 1)
+        self.items, self.ncomp = int(items), int(ncomp)
+        desc = _lib.PlanDesc(ndim=self.ndim, ny=self.ny, nx=self.nx, nt=self.nt, items=self.items,
+                             ncomp=self.ncomp, loss_saturates=int(bool(loss_saturates)), row_lines=0,
+                             col_width=int(col_width))
         h = C.c_void_p()
         _lib.check(_lib.load().nlse_plan_create(C.byref(desc), C.byref(h)), "plan_create")
         self._h = h
@@ -46,6 +51,33 @@ class Plan2D:
             return
         _lib.check(_lib.load().nlse_plan_set_propagator_sep(self._h, px.ctypes.data, py.ctypes.data), "set_prop")
         self._last["prop"] = key
+
+    def set_propagator_c128(self, prop: np.ndarray) -> None:
+        """1-D plans: complex128 propagator [ncomp][nx] (host)."""
+        prop = np.ascontiguousarray(np.asarray(prop, dtype=np.complex128).reshape(self.ncomp, self.nx))
+        key = ("c128", prop.tobytes())
+        if self._last.get("prop") == key:
+            return
+        _lib.check(_lib.load().nlse_plan_set_propagator_c128(self._h, prop.ctypes.data), "set_prop_c128")
+        self._last["prop"] = key
+
+    def set_propagator_t(self, pt: np.ndarray) -> None:
+        """3-D plans: complex128 temporal factor [nt] (host)."""
+        pt = np.ascontiguousarray(np.asarray(pt, dtype=np.complex128).reshape(self.nt))
+        key = pt.tobytes()
+        if self._last.get("prop_t") == key:
+            return
+        _lib.check(_lib.load().nlse_plan_set_propagator_t(self._h, pt.ctypes.data), "set_prop_t")
+        self._last["prop_t"] = key
+
+    def set_propagator_3d_c128(self, table: torch.Tensor) -> None:
+        """3-D plans: dense complex128 propagator (ny, nx, nt) on the device."""
+        t = table.to(torch.complex128).contiguous()
+        if t.numel() != self.ny * self.nx * self.nt:
+            raise ValueError("dense 3-D propagator must have ny*nx*nt points")
+        self._keep["prop"] = t
+        _lib.check(_lib.load().nlse_plan_set_propagator_3d_c128(self._h, t.data_ptr()), "set_prop_3d_c128")
+        self._last["prop"] = ("c128", t.data_ptr())
 
     def set_propagator_c64(self, table: torch.Tensor) -> None:
         t = table.to(torch.complex64).contiguous()
@@ -98,13 +130,13 @@ class Plan2D:
             V = V.to(torch.complex64)
         V = V.contiguous()
         kind = 1 if V.dtype == torch.float32 else 2
-        plane = self.ny * self.nx
+        plane = self.ny * self.nx * self.nt
         if V.numel() == plane:
             stride = 0
         elif V.numel() == plane * self.items:
             stride = plane
         else:
-            raise ValueError(f"V with {V.numel()} points does not match the grid {self.ny}x{self.nx}")
+            raise ValueError(f"V with {V.numel()} points does not match the grid ({self.ny}, {self.nx}, {self.nt})")
         self._keep["V"] = V
         _lib.check(lib.nlse_plan_set_potential(self._h, V.data_ptr(), kind, stride), "set_potential")
 
@@ -126,7 +158,7 @@ class Plan2D:
     def run(self, A: torch.Tensor, nsteps: int, precision: int = 0, step0: int = 0) -> None:
         if A.dtype != torch.complex64 or not A.is_contiguous() or not A.is_cuda:
             raise TypeError("field must be a contiguous complex64 device tensor")
-        if A.numel() != self.items * self.ncomp * self.ny * self.nx:
+        if A.numel() != self.items * self.ncomp * self.ny * self.nx * self.nt:
             raise ValueError("field size does not match the plan")
         _lib.check(_lib.load().nlse_run(self._h, A.data_ptr(), int(nsteps), int(precision), int(step0),
                                         D.stream_ptr()), "run")
@@ -142,8 +174,11 @@ class Plan2D:
         _lib.check(_lib.load().nlse_plan_config(self._h, C.byref(a), C.byref(b)), "plan_config")
         return a.value, b.value
 
-    def __del__(self):
+    def __del__(self):  # noqa: D105
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib._lib is not None:
             _lib._lib.nlse_plan_destroy(h)
             self._h = None
+
+
+Plan2D = Plan

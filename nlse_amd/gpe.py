@@ -1,0 +1,56 @@
+"""GPE on the MI355X engine (reference NLSE/gpe.py:15-119)."""
+
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+from scipy.constants import atomic_mass, c, epsilon_0, hbar
+
+from .nlse import NLSE
+from .utils import __BACKEND__
+
+
+class GPE(NLSE):
+    """A class to solve GPE: the NLSE machinery with (gamma, N, g, m, sat)
+    mapped as in gpe.py:56-79 (k = m through wvl = 2 pi/m, so the nonlinear
+    coefficient is m/2 g c eps0 and V' = m/2 V, quirk Q13)."""
+
+    def __init__(self, gamma: float, N: float, window: float, g: float, V: Union[np.ndarray, None],
+                 m: float = 87 * atomic_mass, NX: int = 1024, NY: int = 1024, sat: float = np.inf,
+                 nl_length: float = 0, backend: str = __BACKEND__) -> None:
+        super().__init__(alpha=gamma, power=N, window=window, n2=g, V=V, L=0, NX=NX, NY=NY, Isat=sat,
+                         nl_length=nl_length, wvl=2 * np.pi / m, backend=backend)
+        self.g = g
+        self.V = V
+        self.gamma = self.alpha
+        self.N = self.power
+        self.m = self.k
+        self.sat = sat
+        self.delta_t = self.delta_z
+        self.I_sat *= epsilon_0 * c / 2
+
+    def _build_propagator(self) -> np.ndarray:
+        """gpe.py:81-90: hbar K^2/2m with delta_t frozen at construction (Q11)."""
+        return np.exp(-1j * 0.5 * hbar * (self.Kxx**2 + self.Kyy**2) / self.m * self.delta_t).astype(np.complex64)
+
+    def _separable_factors(self):
+        px = np.exp(-1j * 0.5 * hbar * self.Kx**2 / self.m * self.delta_t)
+        py = np.exp(-1j * 0.5 * hbar * self.Ky**2 / self.m * self.delta_t)
+        return px[None, :], py[None, :]
+
+    def _prepare_output_array(self, E_in, normalize: bool):
+        """gpe.py:92-119: normalised to N without c eps0/2.  With
+        normalize=False the reference leaves A uninitialised; here A is a
+        copy of E_in (documented deviation Q12)."""
+        E = self._field_to_device(E_in)
+        A = E.clone()
+        import torch
+
+        A_sq = torch.zeros(E.shape, dtype=torch.float32, device=E.device)
+        if normalize:
+            lead = self._lead(E)
+            scale = self._norm_scale(E, self.N, False, lead)
+            isz = int(np.prod(E.shape[len(lead):]))
+            self._scale_items(A, scale, max(1, int(np.prod(lead))), isz, isz)
+        return A, A_sq

@@ -21,7 +21,7 @@ from scipy.constants import c, epsilon_0
 from . import _device as D
 from . import _lib
 from . import kernels_hip
-from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, Plan2D, inv_or_zero
+from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, Plan, inv_or_zero
 from .fft import HipFFTPlan
 from .utils import __BACKEND__
 
@@ -270,15 +270,24 @@ class NLSE:
 
     # ------------------------------------------------------------- engine
     def _ncomp(self) -> int:
+        """Number of field components (CNLSE/DDGPE: 2, layout (..., 2, ...))."""
         return 1
 
-    def _engine_for(self, A: torch.Tensor) -> Plan2D:
+    def _param_lead(self, A) -> tuple:
+        """Leading (realisation) shape of a field, without the component axis."""
         lead = self._lead(A)
-        ncomp = self._ncomp()
-        items = int(np.prod(lead)) // ncomp if lead else 1
-        key = (items, ncomp, self.NY, self.NX, bool(self.loss_saturates), int(self.col_width))
+        return lead[: len(lead) - 1] if self._ncomp() == 2 else lead
+
+    def _engine_geometry(self, A) -> dict:
+        items = int(np.prod(self._param_lead(A))) if self._param_lead(A) else 1
+        return dict(ndim=2, ny=self.NY, nx=self.NX, nt=1, items=items, ncomp=self._ncomp())
+
+    def _engine_for(self, A: torch.Tensor) -> Plan:
+        geo = self._engine_geometry(A)
+        key = tuple(sorted(geo.items())) + (bool(self.loss_saturates), int(self.col_width))
         if self._engine is None or self._engine_key != key:
-            self._engine = Plan2D(self.NY, self.NX, items, ncomp, self.loss_saturates, self.col_width)
+            self._engine = Plan(geo["ny"], geo["nx"], geo["items"], geo["ncomp"], self.loss_saturates,
+                                self.col_width, ndim=geo["ndim"], nt=geo["nt"])
             self._engine_key = key
         return self._engine
 
@@ -296,20 +305,34 @@ class NLSE:
         rec["vdz"] = dz * (self.k / 2)
         return rec
 
-    def _configure(self, eng: Plan2D, A: torch.Tensor, V, propagator, precision: str) -> None:
-        lead = self._lead(A)
+    def _nl_slots(self, precision: str, lead: tuple, V=None) -> dict:
+        """Nonlinear applications of a step: 'single' N(dz) after the linear
+        step; 'double' N(dz/2) before and after (nlse.py:404-487)."""
         dz = self.delta_z
         if precision == "double":
             half = self._nl_records(dz / 2, lead)
-            eng.set_nl1(SLOT_HALF_PRE, half)
-            eng.set_nl1(SLOT_HALF_POST, half)
-        else:
-            eng.set_nl1(SLOT_FULL, self._nl_records(dz, lead))
+            return {SLOT_HALF_PRE: half, SLOT_HALF_POST: half}
+        return {SLOT_FULL: self._nl_records(dz, lead)}
+
+    def _rabi_params(self, lead: tuple):
+        return None
+
+    def _configure(self, eng: Plan, A: torch.Tensor, V, propagator, precision: str) -> None:
+        if self.nl_length > 0:
+            self._configure_nonlocal(eng)
+        lead = self._param_lead(A)
+        setter = eng.set_nl1 if self._ncomp() == 1 else eng.set_nl2
+        for slot, rec in self._nl_slots(precision, lead, V).items():
+            setter(slot, rec)
+        if self._ncomp() == 2:
+            r = self._rabi_params(lead)
+            eng.set_rabi(*(r if r is not None else (None, None)))
         self._configure_tables(eng, V, propagator)
 
-    def _configure_tables(self, eng: Plan2D, V, propagator) -> None:
-        if self.nl_length > 0:
-            raise NotImplementedError("nonlocal response is not available in this build of the engine")
+    def _configure_nonlocal(self, eng: Plan) -> None:
+        raise NotImplementedError("nonlocal response is not available in this build of the engine")
+
+    def _configure_tables(self, eng: Plan, V, propagator) -> None:
         pf = self._prop_factors
         if pf is not None and pf[0] is propagator:
             eng.set_propagator_sep(pf[1], pf[2])
@@ -372,11 +395,12 @@ class NLSE:
         t0 = time.perf_counter()
         if isinstance(self.delta_z, complex):
             raise NotImplementedError("imaginary-time evolution (complex delta_z) is not supported")
+        callback, callback_args, fused = self._fuse_callbacks(callback, callback_args)
         if callback is None:
             nsteps = count_steps(z, self.delta_z)
             if nsteps and z > self.L:
                 self.n2 = 0
-            self._run_chunk(A, V, nsteps, precision, 0)
+            self._run_fused(A, V, nsteps, precision, fused)
         else:
             self._loop_with_callbacks(A, A_sq, V, z, precision, verbose, callback, callback_args)
         end.record()
@@ -392,6 +416,14 @@ class NLSE:
         if plot:
             self.plot_field(A, z)
         return A
+
+    def _fuse_callbacks(self, callback, callback_args):
+        """Callbacks that the engine runs inside the fused sweeps (none for
+        NLSE); returns (host callbacks or None, their args, engine setup)."""
+        return callback, callback_args, None
+
+    def _run_fused(self, A, V, nsteps: int, precision: str, fused) -> None:
+        self._run_chunk(A, V, nsteps, precision, 0)
 
     def _loop_with_callbacks(self, A, A_sq, V, z, precision, verbose, callback, callback_args) -> None:
         """The reference loop body (nlse.py:566-588), one step at a time."""

@@ -1,0 +1,47 @@
+"""NLSE_1d on the MI355X engine (reference NLSE/nlse_1d.py:15-105).
+
+One launch runs all steps of every line (the line lives in registers and
+LDS for the whole propagation)."""
+
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+import torch
+from scipy.constants import c, epsilon_0
+
+from . import _device as D
+from .nlse import NLSE
+from .utils import __BACKEND__
+
+
+class NLSE_1d(NLSE):
+    """A class to solve NLSE in 1d."""
+
+    def __init__(self, alpha: float, power: float, window: float, n2: float, V: Union[np.ndarray, None],
+                 L: float, NX: int = 1024, Isat: float = np.inf, nl_length: float = 0, wvl: float = 780e-9,
+                 backend: str = __BACKEND__) -> None:
+        super().__init__(alpha=alpha, power=power, window=window, n2=n2, V=V, L=L, NX=NX, Isat=Isat,
+                         nl_length=nl_length, wvl=wvl, backend=backend)
+        self._last_axes = (-1,)
+        self.nl_profile = self.nl_profile[0]
+        self.nl_profile /= self.nl_profile.sum()
+
+    def _cell(self) -> float:
+        return self.delta_X**2  # nlse_1d.py:89 (dX^2, quirk Q9)
+
+    def _build_propagator(self) -> np.ndarray:
+        """nlse_1d.py:98-105: complex128 (quirk Q8)."""
+        return np.exp(-1j * 0.5 * (self.Kx**2) / self.k * self.delta_z)
+
+    def _separable_factors(self):
+        return None
+
+    def _engine_geometry(self, A) -> dict:
+        lead = self._param_lead(A)
+        return dict(ndim=1, ny=1, nx=self.NX, nt=1, items=int(np.prod(lead)) if lead else 1, ncomp=self._ncomp())
+
+    def _configure_tables(self, eng, V, propagator) -> None:
+        eng.set_propagator_c128(np.asarray(D.to_host(propagator), dtype=np.complex128))
+        eng.set_potential(None if V is None else D.to_device(V))

@@ -1,0 +1,90 @@
+"""NLSE_3d on the MI355X engine (reference NLSE/nlse_3d.py:15-150).
+
+Field layout (..., NY, NX, NT), t fastest.  One step is four sweeps: the
+y columns carry the propagator (complex64 2-D part times the complex128
+temporal factor, FP64 product), the t rows the nonlinear step."""
+
+from __future__ import annotations
+
+from typing import Union
+
+import numpy as np
+import torch
+
+from . import _device as D
+from .nlse import NLSE
+from .utils import __BACKEND__
+
+
+class NLSE_3d(NLSE):
+    """A class to solve the 3D NLSE i.e propagation of pulses of light in
+    nonlinear media."""
+
+    def __init__(self, alpha: float, energy: float, window: Union[list, tuple], n2: float, D0: float, vg: float,
+                 V: Union[np.ndarray, None], L: float, NX: int = 1024, NY: int = 1024, NZ: int = 1024,
+                 Isat: float = np.inf, nl_length: float = 0, wvl: float = 780e-9,
+                 backend: str = __BACKEND__) -> None:
+        if len(window) == 2:
+            window = [window[0], window[0], window[-1]]
+        super().__init__(alpha=alpha, power=energy, window=window[0:2], n2=n2, V=V, L=L, NX=NX, NY=NY,
+                         Isat=Isat, nl_length=nl_length, wvl=wvl, backend=backend)
+        self.energy = self.power
+        self.NZ = NZ
+        self.window_t = window[-1]
+        self.power = self.energy / self.window_t
+        Dn = self.n2 * self.power / min(self.window[0:2]) ** 2
+        z_nl = 1 / (self.k * abs(Dn))
+        if isinstance(z_nl, np.ndarray):
+            z_nl = z_nl.min()
+        self.delta_z = 0.5e-2 * z_nl
+        self.T, self.delta_T = np.linspace(-self.window_t / 2, self.window_t / 2, self.NZ, retstep=True)
+        self.omega = 2 * np.pi * np.fft.fftfreq(self.NZ, self.delta_T)
+        self.D0 = D0
+        self.vg = vg
+        self.XX, self.YY, self.TT = np.meshgrid(self.X, self.Y, self.T)
+        self.Kxx, self.Kyy, self.Omega = np.meshgrid(self.Kx, self.Ky, self.omega)
+        self._last_axes = (-3, -2, -1)
+
+    def _build_propagator(self) -> np.ndarray:
+        """nlse_3d.py:105-114: complex64 2-D part times complex128
+        exp(-i D0/2 Omega^2), which has no dz factor (quirk Q6)."""
+        prop_2d = np.exp(-1j * 0.5 * (self.Kxx**2 + self.Kyy**2) / self.k * self.delta_z).astype(np.complex64)
+        prop_t = np.exp(-1j * self.D0 / 2 * self.Omega**2)
+        return prop_2d * prop_t
+
+    def _separable_factors(self):
+        px = np.exp(-1j * 0.5 * self.Kx**2 / self.k * self.delta_z)
+        py = np.exp(-1j * 0.5 * self.Ky**2 / self.k * self.delta_z)
+        return px[None, :], py[None, :]
+
+    def _own_propagator(self) -> None:
+        super()._own_propagator()
+        self._prop_t = np.exp(-1j * self.D0 / 2 * self.omega**2)
+
+    def _cell(self) -> float:
+        return self.delta_X * self.delta_Y * self.delta_T
+
+    def _prepare_output_array(self, E_in, normalize: bool):
+        """nlse_3d.py:116-150: normalised to the energy."""
+        E = self._field_to_device(E_in)
+        A = E.clone()
+        A_sq = torch.zeros(E.shape, dtype=torch.float32, device=E.device)
+        if normalize:
+            lead = self._lead(E)
+            scale = self._norm_scale(E, self.energy, True, lead)
+            isz = int(np.prod(E.shape[len(lead):]))
+            self._scale_items(A, scale, max(1, int(np.prod(lead))), isz, isz)
+        return A, A_sq
+
+    def _engine_geometry(self, A) -> dict:
+        lead = self._param_lead(A)
+        return dict(ndim=3, ny=self.NY, nx=self.NX, nt=self.NZ, items=int(np.prod(lead)) if lead else 1, ncomp=1)
+
+    def _configure_tables(self, eng, V, propagator) -> None:
+        pf = self._prop_factors
+        if pf is not None and pf[0] is propagator:
+            eng.set_propagator_sep(pf[1], pf[2])
+            eng.set_propagator_t(self._prop_t)
+        else:
+            eng.set_propagator_3d_c128(D.to_device(propagator, torch.complex128))
+        eng.set_potential(None if V is None else D.to_device(V))

@@ -53,9 +53,9 @@ def test_argument_errors_without_gpu():
     assert rc == -1 and b"unsupported" in lib.nlse_last_error()
     rc = lib.nlse_fft_plan_create(4, dims, 1, C.byref(h))
     assert rc == -1
-    desc = _lib.PlanDesc(3, 64, 64, 1, 1, 0, 0, 0)
+    desc = _lib.PlanDesc(ndim=4, ny=64, nx=64, nt=1, items=1, ncomp=1)
     assert lib.nlse_plan_create(C.byref(desc), C.byref(h)) == -1
-    desc = _lib.PlanDesc(2, 64, 64, 1, 3, 0, 0, 0)
+    desc = _lib.PlanDesc(ndim=2, ny=64, nx=64, nt=1, items=1, ncomp=3)
     assert lib.nlse_plan_create(C.byref(desc), C.byref(h)) == -1
     assert lib.nlse_run(None, None, 1, 0, 0, None) == -1
     assert lib.nlse_square_mod(None, None, 4, None) == -1
