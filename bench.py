@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: split-steps/s of the fused 2-D split step at 2048^2 complex64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--local]
 
 A step is one split step (linear step in Fourier space + real-space
-nonlinear step) of the 2048x2048 complex64 field of the config-1 physics
-(alpha=20, n2=-1.6e-9, Isat=1e5, real potential V, dz=5e-6, speckle input),
-with the field resident in HBM.  The run is the fused chain of the engine:
-K steps = one column sweep and one row sweep per step.
+nonlinear step) of the 2048x2048 complex64 field of the config-1 workload:
+alpha=20, n2=-1.6e-9, Isat=1e5, real potential V, dz=5e-6, speckle input,
+and the nonlocal response of nl_length = 20 um (the reference's K0 kernel:
+d = nl_length // dX = 4 points, a 25x25 kernel applied as a zero-padded
+linear convolution), with the field resident in HBM.  ``--local`` drops the
+nonlocal term (two sweeps per step).
 
 Printed: one JSON line with the throughput, the roofline of the dominant
 kernel (its average duration measured with HIP events on the engine's
-stream), the step-level roofline, and the CPU baseline (the oracle's
-complex64 path on the host cores, a bounded sample of the same workload).
+stream), the per-kernel and step-level byte counts, and the CPU baseline
+(the oracle's complex64 path on the host cores, a bounded sample of the
+same workload).
 """
 
 from __future__ import annotations
@@ -27,16 +30,19 @@ import time
 import numpy as np
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
-PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "split_step_2048_c64.md")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+PROFILE = {False: os.path.join(PROFILES, "split_step_nonlocal_2048_c64.md"),
+           True: os.path.join(PROFILES, "split_step_2048_c64.md")}
 
 
-def measured_traffic(sweep: str, grid: int):
-    """HBM bytes per launch of a sweep from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (2048^2 workload only), else None."""
-    if grid != 2048 or not os.path.exists(PROFILE):
+def measured_traffic(sweep: str, grid: int, local: bool):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (2048^2 workloads only), else None."""
+    path = PROFILE[local]
+    if grid != 2048 or not os.path.exists(path):
         return None
     try:
-        text = open(PROFILE).read()
+        text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
         return float(json.loads(block)[sweep]["traffic_bytes"])
     except (IndexError, KeyError, ValueError):
@@ -54,14 +60,16 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(N: int, budget_s: float, threads: int) -> dict:
+def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float) -> dict:
     """Oracle (complex64, scipy single-precision FFT, elementwise steps on a
-    thread pool) on the host cores: steps of the same 2048^2 workload until
-    `budget_s` of wall time has passed."""
+    thread pool, scipy oaconvolve for the nonlocal term) on the host cores:
+    steps of the same 2048^2 workload until `budget_s` of wall time has
+    passed."""
     import oracle
     from nlse_amd import workloads as W
 
-    sim = W.config1(oracle.NLSE, N, nonlocal_length=0, fft="pocketfft32", workers=threads, threads=threads)
+    sim = W.config1(oracle.NLSE, N, nonlocal_length=nl_length, fft="pocketfft32", workers=threads,
+                    threads=threads)
     E0 = W.config1_field(sim)
     A, A_sq = sim._prepare_output_array(E0, True)
     prop = sim._build_propagator()
@@ -79,19 +87,39 @@ def cpu_baseline(N: int, budget_s: float, threads: int) -> dict:
         "unit": "split-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} split steps of the {N}x{N} config-1 workload (without the nonlocal term) on the "
+        "sample": f"{n} split steps of the {N}x{N} config-1 workload "
+                  f"({'nonlocal, scipy oaconvolve' if nl_length > 0 else 'without the nonlocal term'}) on the "
                   f"CPU oracle: complex64, scipy.fft single precision with {threads} workers, elementwise "
                   f"steps on {threads} threads; {el:.1f} s; host CPU: {cpu_model()} "
                   f"(os.cpu_count()={os.cpu_count()})",
     }
 
 
+def kernel_bytes(P: int, N: int, r: int, local: bool) -> dict:
+    """Algorithmic HBM bytes per launch of each kernel of the step (P field
+    points, complex64 field, float32 V).  Nonlocal one-component fields use
+    the packed intensity torus of P/2 complex points (4 B per field point)
+    and band points within r of its edges."""
+    if local:
+        return {"row": (0, P * (16 + 4)), "col": (1, P * 16)}
+    band = 4 * r * N + (N - 4 * r) * 2 * r
+    khat = 4 * (N // 4 + 1) * (N // 2 + 1)
+    return {
+        "col": (1, P * 16),                               # field F_y, P, I_y
+        "nlc_z": (2, P * (8 + 8 + 4)),                    # field in/out, Z out
+        "band": (3, band * (8 + 4)),                      # field at band points, values out
+        "colZ": (4, P * 8 + khat),                        # Z in/out, kernel spectrum
+        "nlc_nl": (5, P * (8 + 4 + 4 + 8) + band * 4),    # field, Z, V in; field out
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--local", action="store_true", help="drop the nonlocal term (config-1 physics otherwise)")
     ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -107,7 +135,8 @@ def main() -> int:
     from nlse_amd import workloads as W
 
     N = args.grid
-    sim = W.config1(NLSE, N, nonlocal_length=0)
+    nl_length = 0.0 if args.local else 20e-6
+    sim = W.config1(NLSE, N, nonlocal_length=nl_length)
     sim.col_width = args.col_width
     E0 = W.config1_field(sim)
     sim._own_propagator()
@@ -116,10 +145,9 @@ def main() -> int:
     eng = sim._engine_for(A)
     sim._configure(eng, A, sim.V, sim.propagator, "single")
     P = A.numel()
-    vbytes = 4  # float32 V read once per row sweep
-    bytes_row = P * (16 + vbytes)
-    bytes_col = P * 16
-    step_bytes = P * (32 + vbytes)
+    r = 0 if args.local else int(tuple(sim.nl_profile.shape)[0] // 2)
+    kb = kernel_bytes(P, N, r, args.local)
+    step_bytes = sum(b for _, b in kb.values())
 
     # warm-up, then K steps bracketed by synchronisation
     eng.run(A, args.warmup)
@@ -134,10 +162,12 @@ def main() -> int:
     steps_per_s = args.steps / t_steps
     finite = bool(torch.isfinite(torch.view_as_real(A)).all())
 
-    # each sweep alone, same stream, same launch configuration
+    # each kernel alone, same stream, same launch configuration
     scratch = A.clone()
+    if not args.local:
+        eng.run(scratch, 1)  # fills the intensity and band buffers
     kern = {}
-    for name, which, nbytes in (("row", 0, bytes_row), ("col", 1, bytes_col)):
+    for name, (which, nbytes) in kb.items():
         eng.run_pass(scratch, which, 10)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -152,6 +182,14 @@ def main() -> int:
     achieved = kern[dom]["bytes"] / (kern[dom]["avg_us"] * 1e-6)
     row_lines, col_width = eng.config()
 
+    physics = ("real float32 potential, Isat=1e5, alpha=20, dz=5e-6, speckle input (seed 1), "
+               "precision='single'")
+    if args.local:
+        workload = f"config-1 physics without the nonlocal term: {N}x{N} complex64 NLSE split step, " + physics
+    else:
+        workload = (f"config-1: {N}x{N} complex64 NLSE split step with the nonlocal response "
+                    f"(nl_length 20 um, reference K0 kernel {2 * r + 1}x{2 * r + 1}, zero-padded linear "
+                    "convolution), " + physics)
     result = {
         "metric": "split-steps/s at 2048^2 c64",
         "value": steps_per_s,
@@ -166,20 +204,19 @@ def main() -> int:
         "dtype": "c64",
         "data": "synthetic",
         "config": {
-            "workload": f"config-1 physics without the nonlocal term: {N}x{N} complex64 NLSE split step, "
-                        "real float32 potential, Isat=1e5, alpha=20, dz=5e-6, speckle input (seed 1), "
-                        "precision='single'",
+            "workload": workload,
             "grid": [N, N],
+            "nonlocal_kernel": None if args.local else [2 * r + 1, 2 * r + 1],
             "kernels": {"row_lines_per_block": row_lines, "col_width": col_width},
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{dom} sweep",
+            "kernel": dom,
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": measured_traffic(dom, N),
+            "traffic": measured_traffic(dom, N, args.local),
         },
         "kernels": kern,
         "step_roofline": {
@@ -187,13 +224,15 @@ def main() -> int:
             "B_per_point": step_bytes / P,
             "achieved_GBps": steps_per_s * step_bytes / 1e9,
             "frac": steps_per_s * step_bytes / HBM_PEAK,
+            "min_B_per_point": 36 if args.local else 52,
+            "frac_of_min_schedule": steps_per_s * P * (36 if args.local else 52) / HBM_PEAK,
         },
         "grid_point_steps_per_s": steps_per_s * P,
         "field_finite": finite,
     }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads)
+        result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads, nl_length)
     print(json.dumps(result), flush=True)
     return 0
 

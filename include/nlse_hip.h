@@ -181,6 +181,14 @@ int nlse_plan_set_potential(nlse_plan* plan, const void* V_dev, int v_kind, int6
  * the absolute step number; n_t their length.  NULL disables. */
 int nlse_plan_set_drive(nlse_plan* plan, const void* Fr0_dev, const void* Fr1_dev,
                         const float* Ft0, const float* Ft1, int64_t n_t, double dz);
+/* Nonlocal response (replaces the nl_profile convolution, nlse.py:143-153,
+ * 406-409, 444-447; cnlse.py:243-249, 317-323; nlse_1d.py:65-66): every
+ * nonlinear application uses the zero-padded linear "same" convolution of
+ * |A|^2 with `kernel` (host float64, row-major kh x kw, both odd; 1-D plans
+ * take kh = 1).  2-D kernels must be square and even in both axes (the K0
+ * profile is).  NULL disables.  3-D plans reject it (the reference's
+ * oaconvolve rejects a 2-D kernel on a 3-D field). */
+int nlse_plan_set_nonlocal(nlse_plan* plan, const double* kernel, int64_t kh, int64_t kw);
 /* Run nsteps split steps on A (device, real space in and out).
  * precision: 0 "single" (Lie), 1 "double" (Strang).  step0 is the absolute
  * index of the first step (drive time index). */
@@ -188,8 +196,10 @@ int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_
              void* stream);
 /* Launch one sweep type `count` times in place on A (device): pass 0 is a
  * steady-state row sweep (I_x, nonlinear step, F_x), pass 1 a column sweep
- * (F_y, propagator, I_y).  The data is meaningless afterwards; this exists
- * to time each kernel of the step in isolation. */
+ * (F_y, propagator, I_y); with a nonlocal response, passes 2-5 are its
+ * launches: 2 I_x + intensity transform, 3 band convolution, 4 intensity
+ * column sweep, 5 nonlinear step + F_x.  The data is meaningless
+ * afterwards; this exists to time each kernel of the step in isolation. */
 int nlse_run_pass(nlse_plan* plan, void* A_dev, int pass, int64_t count, int precision, void* stream);
 /* Kernel configuration actually used (rows per block, columns per block). */
 int nlse_plan_config(const nlse_plan* plan, int* row_lines, int* col_width);

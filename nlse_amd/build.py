@@ -2,6 +2,9 @@
 
     python -m nlse_amd.build        # compile (always recompiles)
 
+The host/ABI unit and the per-line-length kernel units are compiled in
+parallel into nlse_amd/_build/ and linked into nlse_amd/libnlse_hip.so.
+
 Packed FP32 VALU instructions are disabled (``-packed-fp32-ops``): the
 engine's FP32 arithmetic is issued as ordinary VOP3 instructions.
 """
@@ -12,9 +15,14 @@ import os
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "engine.hip")
+CSRC = os.path.join(HERE, "csrc")
+# the per-line-length kernel units compile in parallel with the host unit
+UNITS = ["engine.hip", "lines_small.hip", "lines_512.hip", "lines_1024.hip", "lines_2048.hip",
+         "lines_4096.hip"]
+OBJDIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libnlse_hip.so")
 ARCH = os.environ.get("NLSE_OFFLOAD_ARCH", "gfx950")
 
@@ -23,29 +31,44 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
+# hipcc's host pass does not know the packed-fp32 target feature: harmless
+NOISE = ("packed-fp32-ops", "is not a recognized feature")
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in res.stderr.splitlines() if not any(n in l for n in NOISE))
+    return res.returncode, err
+
 
 def build(verbose: bool = True) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc, *FLAGS, "-o", tmp, SRC]
+    os.makedirs(OBJDIR, exist_ok=True)
     t0 = time.time()
+    jobs = []
+    for unit in UNITS:
+        obj = os.path.join(OBJDIR, unit.replace(".hip", ".o"))
+        jobs.append((obj, [hipcc, *FLAGS, "-c", "-o", obj, os.path.join(CSRC, unit)]))
     if verbose:
-        print("[nlse_amd.build]", " ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    # hipcc's host pass does not know the packed-fp32 target feature: harmless
-    noise = ("packed-fp32-ops", "is not a recognized feature")
-    err = "\n".join(l for l in res.stderr.splitlines() if not any(n in l for n in noise))
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
+        print("[nlse_amd.build]", " ".join(jobs[0][1]), f"(+{len(jobs) - 1} units in parallel)", flush=True)
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(workers) as ex:
+        results = list(ex.map(lambda j: _run(j[1]), jobs))
+    for (obj, _), (rc, err) in zip(jobs, results):
+        if rc != 0:
+            raise RuntimeError(f"hipcc failed on {os.path.basename(obj)} ({rc}):\n{err}")
+        if verbose and err.strip():
+            print(err, file=sys.stderr)
+    tmp = OUT + ".tmp"
+    rc, err = _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[o for o, _ in jobs]])
+    if rc != 0:
+        raise RuntimeError(f"link failed ({rc}):\n{err}")
     os.replace(tmp, OUT)
     if verbose:
-        if err.strip():
-            print(err, file=sys.stderr)
         print(f"[nlse_amd.build] built {OUT} in {time.time() - t0:.1f}s", flush=True)
     return OUT
 

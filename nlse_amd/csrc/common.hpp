@@ -1,0 +1,35 @@
+// Shared host-side helpers of libnlse_hip.so: error reporting (the message
+// returned by nlse_last_error) and launch checks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/nlse_hip.h"
+
+namespace nlse {
+
+// Message of the last failed call on this thread (defined in engine.hip).
+std::string& error_buffer();
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(NLSE_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return NLSE_OK;
+}
+
+inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace nlse
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::nlse::fail(NLSE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)

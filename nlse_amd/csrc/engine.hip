@@ -2,6 +2,7 @@
 // (declared in include/nlse_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -12,223 +13,36 @@
 #include <vector>
 
 #include "../../include/nlse_hip.h"
-#include "passes_2d.hpp"
-
-#ifndef NLSE_COL_LDS_KB
-#define NLSE_COL_LDS_KB 150  // LDS budget of one column-sweep block
-#endif
+#include "band.hpp"
+#include "common.hpp"
+#include "lines.hpp"
 
 using namespace nlse;
 
 // ------------------------------------------------------------ errors
-static thread_local std::string g_err;
+namespace nlse {
 
-static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int fail(int code, const char* fmt, ...) {
+std::string& error_buffer() {
+  static thread_local std::string buf;
+  return buf;
+}
+
+int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_err = buf;
+  error_buffer() = buf;
   return code;
 }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(NLSE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
-  } while (0)
-
-static inline int launch_status(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(NLSE_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
-  return NLSE_OK;
-}
-
-static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
-
-// ------------------------------------------------------ twiddle tables
-// One table per (device, line length), layout of Plan<N,P>::tw_offset:
-// stage s (s >= 1), entry [(k-1)*Ns + jm] = exp(-2 pi i jm k / (Ns R)),
-// evaluated in FP64 and stored as {c_hi, s_hi, c_lo, s_lo} with
-// c = c_hi + c_lo to ~2^-48.
-template <int N, int P>
-static std::vector<float4> build_twiddles() {
-  using PL = Plan<N, P>;
-  std::vector<float4> tw(PL::TW_SIZE > 0 ? PL::TW_SIZE : 1);
-  for (int s = 1; s < PL::NSTAGES; ++s) {
-    const int R = PL::radix(s), NS = PL::ns(s), off = PL::tw_offset(s);
-    for (int k = 1; k < R; ++k)
-      for (int jm = 0; jm < NS; ++jm) {
-        // angle = -2 pi * jm*k/(NS*R); reduce the rational first
-        const long num = (long)jm * k, den = (long)NS * R;
-        const double ang = -2.0 * M_PI * (double)num / (double)den;
-        const double c = std::cos(ang), sn = std::sin(ang);
-        const float ch = (float)c, sh = (float)sn;
-        tw[off + (k - 1) * NS + jm] = make_float4(ch, sh, (float)(c - (double)ch), (float)(sn - (double)sh));
-      }
-  }
-  return tw;
-}
-
-static std::mutex g_tw_mu;
-static std::map<std::pair<int, int>, float4*> g_tw;  // (device, N) -> table
-
-template <int N, int P>
-static int get_twiddles(const float4** out) {
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto key = std::make_pair(dev, N);
-  auto it = g_tw.find(key);
-  if (it != g_tw.end()) {
-    *out = it->second;
-    return NLSE_OK;
-  }
-  std::vector<float4> h = build_twiddles<N, P>();
-  float4* d = nullptr;
-  HIP_TRY(hipMalloc(&d, h.size() * sizeof(float4)));
-  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
-  g_tw[key] = d;
-  *out = d;
-  return NLSE_OK;
-}
-
-// -------------------------------------------------- kernel dispatchers
-template <int N>
-struct LineCfg {
-  static constexpr int P = N >= 16 ? 16 : N;
-  static constexpr int T = N / P;
-  static constexpr int LINE = LdsLine<N, 3, 3>::SIZE;  // column-pass lines
-  // rows per block: 256 threads for one component, 128 for pairs
-  static constexpr int LPB1 = (256 / T) >= 1 ? (256 / T) : 1;
-  static constexpr int LPB2 = (128 / T) >= 1 ? (128 / T) : 1;
-  // default column tile: 8 columns (64-byte row segments; wider tiles
-  // spill registers at 512 threads)
-  static constexpr int WDEF = 8;
-};
-
-// Column tile of W columns: WT threads across (<= 512 threads per block,
-// i.e. up to 256 VGPRs per lane), G = W/WT columns per thread, LDS rounds
-// of GR columns per thread column (<= 150 KiB of LDS).
-template <int N, int W>
-struct ColCfg {
-  using C = LineCfg<N>;
-  static constexpr int WTMAX = (512 / C::T) >= 1 ? (512 / C::T) : 1;
-  static constexpr int WT = W < WTMAX ? W : WTMAX;
-  static constexpr int G = W / WT;
-  static constexpr int GRMAX = (NLSE_COL_LDS_KB * 1024) / (WT * C::LINE * 8);
-  static constexpr int gr(int g) { return (g <= GRMAX || g == 1) ? g : gr(g / 2); }
-  static constexpr int GR = gr(G);
-  static constexpr bool OK = G >= 1 && G <= 4 && GR >= 1 && WT * GR * C::LINE * 8 <= 160 * 1024;
-};
-
-template <int N>
-static int row_launch(int G, const RowArgs& a, hipStream_t s) {
-  using C = LineCfg<N>;
-  if (G == 1) {
-    constexpr int L = C::LPB1;
-    const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_row<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  } else {
-    constexpr int L = C::LPB2;
-    const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_row<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  }
-  return launch_status("k_row");
-}
-
-template <int N>
-static int line1d_launch(int G, const Line1DArgs& a, hipStream_t s) {
-  using C = LineCfg<N>;
-  if (G == 1) {
-    constexpr int L = C::LPB1;
-    const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_line1d<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  } else {
-    constexpr int L = C::LPB2;
-    const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_line1d<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  }
-  return launch_status("k_line1d");
-}
-
-template <int N, int W>
-static bool col_launch_w(const ColArgs& a, hipStream_t s) {
-  using C = LineCfg<N>;
-  using K = ColCfg<N, W>;
-  if constexpr (K::OK) {
-    const int64_t blocks = a.nslabs * (a.NX / W);
-    hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR>), dim3((unsigned)blocks), dim3(K::WT * C::T), 0, s,
-                       a);
-    return true;
-  } else {
-    return false;
-  }
-}
-
-template <int N>
-static bool col_ok(int w) {
-  switch (w) {
-    case 1: return ColCfg<N, 1>::OK;
-    case 2: return ColCfg<N, 2>::OK;
-    case 4: return ColCfg<N, 4>::OK;
-    case 8: return ColCfg<N, 8>::OK;
-    case 16: return ColCfg<N, 16>::OK;
-    case 32: return ColCfg<N, 32>::OK;
-    case 64: return ColCfg<N, 64>::OK;
-    default: return false;
-  }
-}
-
-template <int N>
-static int col_width_for(int64_t ncols, int want) {
-  int w = want > 0 ? want : LineCfg<N>::WDEF;
-  if (w > 64) w = 64;
-  while (w > 1 && (w > ncols || (ncols % w) != 0 || !col_ok<N>(w))) w >>= 1;
-  return w;
-}
-
-template <int N>
-static int col_launch(int W, const ColArgs& a, hipStream_t s) {
-  bool ok = false;
-  switch (W) {
-    case 1: ok = col_launch_w<N, 1>(a, s); break;
-    case 2: ok = col_launch_w<N, 2>(a, s); break;
-    case 4: ok = col_launch_w<N, 4>(a, s); break;
-    case 8: ok = col_launch_w<N, 8>(a, s); break;
-    case 16: ok = col_launch_w<N, 16>(a, s); break;
-    case 32: ok = col_launch_w<N, 32>(a, s); break;
-    case 64: ok = col_launch_w<N, 64>(a, s); break;
-    default: break;
-  }
-  if (!ok) return fail(NLSE_ERR_ARG, "unsupported column width %d for length %d", W, N);
-  return launch_status("k_col");
-}
-
-// Runtime-N dispatch table.
-struct LineOps {
-  int (*twiddles)(const float4**);
-  int (*row)(int, const RowArgs&, hipStream_t);
-  int (*line1d)(int, const Line1DArgs&, hipStream_t);
-  int (*col)(int, const ColArgs&, hipStream_t);
-  int (*colw)(int64_t, int);
-  int lpb1, lpb2, t;
-};
-
-template <int N>
-static LineOps make_ops() {
-  using C = LineCfg<N>;
-  return LineOps{&get_twiddles<N, C::P>, &row_launch<N>, &line1d_launch<N>, &col_launch<N>, &col_width_for<N>,
-                 C::LPB1, C::LPB2, C::T};
-}
+}  // namespace nlse
 
 static const LineOps* line_ops(int64_t n) {
-  static const LineOps ops[] = {make_ops<8>(),   make_ops<16>(),  make_ops<32>(),  make_ops<64>(),
-                                make_ops<128>(), make_ops<256>(), make_ops<512>(), make_ops<1024>(),
-                                make_ops<2048>(), make_ops<4096>()};
+  static const LineOps ops[] = {make_ops_8(),   make_ops_16(),  make_ops_32(),  make_ops_64(),
+                                make_ops_128(), make_ops_256(), make_ops_512(), make_ops_1024(),
+                                make_ops_2048(), make_ops_4096()};
   if (!is_pow2(n) || n < 8 || n > 4096) return nullptr;
   int l = 0;
   while ((int64_t(1) << l) < n) ++l;
@@ -255,7 +69,7 @@ static ColArgs col_defaults() {
 extern "C" {
 
 int nlse_version(void) { return 100; }
-const char* nlse_last_error(void) { return g_err.c_str(); }
+const char* nlse_last_error(void) { return nlse::error_buffer().c_str(); }
 
 // ---------------------------------------------------------------- FFT
 struct nlse_fft_plan {
@@ -637,7 +451,40 @@ struct nlse_plan {
   std::vector<float2> ft0, ft1;
   float drv_dz = 0.f;
   bool drive = false;
+  // nonlocal response
+  struct {
+    bool on = false;
+    int r = 0;                      // kernel half-width (after trimming)
+    int64_t M = 0;                  // torus rows of the packed intensity
+    bool direct = false;            // every point convolved directly
+    const LineOps* zops = nullptr;  // columns of length M
+    const float4* twz = nullptr;
+    int colw = 1;
+    float2* Z = nullptr;            // [items][M][nx]
+    float* kq = nullptr;            // [M/2+1][nx/2+1] spectrum / (M nx)
+    double* K = nullptr;            // 2-D: [kr][2r+1]; 1-D: [2r+1]
+    int kr = 0, kc = 0;
+    int* rowmap = nullptr;          // [ny]
+    int nb = 0;
+    BandTile* tiles = nullptr;
+    int ntiles = 0;
+    size_t lds = 0;
+    float* rowband = nullptr;
+    float* colband = nullptr;
+  } nlc;
 };
+
+static void nlc_free(nlse_plan* p) {
+  auto& n = p->nlc;
+  (void)hipFree(n.Z);
+  (void)hipFree(n.kq);
+  (void)hipFree(n.K);
+  (void)hipFree(n.rowmap);
+  (void)hipFree(n.tiles);
+  (void)hipFree(n.rowband);
+  (void)hipFree(n.colband);
+  n = {};
+}
 
 int nlse_plan_create(const nlse_plan_desc* d, nlse_plan** out) {
   if (!out || !d) return fail(NLSE_ERR_ARG, "null argument");
@@ -680,6 +527,7 @@ int nlse_plan_destroy(nlse_plan* p) {
   (void)hipFree(p->prop128);
   (void)hipFree(p->pt);
   (void)hipFree(p->rabi);
+  nlc_free(p);
   delete p;
   return NLSE_OK;
 }
@@ -812,6 +660,150 @@ int nlse_plan_set_drive(nlse_plan* p, const void* fr0, const void* fr1, const fl
   return NLSE_OK;
 }
 
+// LDS bytes of one k_nl_band block for tiles of `nc` columns.
+static size_t band_lds(int nc, int r, int kc) {
+  return band_img_floats(nc, r, kc) * sizeof(float) + (size_t)kc * (2 * r + 1) * sizeof(double) +
+         (size_t)BAND_WAVES * 64 * RB_BAND * sizeof(double);
+}
+
+// Band geometry and kernel spectrum of a 2-D plan (see nonlocal.hpp).
+static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
+  auto& n = p->nlc;
+  const int64_t ny = p->d.ny, nx = p->d.nx, items = p->d.items, slabs = items * p->d.ncomp;
+  const int r0 = kw / 2;
+  for (int u = 0; u < kw; ++u)
+    for (int v = 0; v < kw; ++v) {
+      const double k = kernel[u * kw + v];
+      if (k != kernel[(kw - 1 - u) * kw + v] || k != kernel[u * kw + (kw - 1 - v)])
+        return fail(NLSE_ERR_ARG, "nonlocal kernel must be even in both axes");
+    }
+  // taps farther than the field extent only ever meet the zero padding
+  const int r = (int)std::min<int64_t>(r0, std::max(ny, nx) - 1);
+  const int w = 2 * r + 1;
+  n.r = r;
+  n.M = p->d.ncomp == 1 ? ny / 2 : ny;
+  n.direct = n.M < 8 || 2 * r >= n.M || 2 * r >= nx;
+  // band rows in increasing order
+  std::vector<int> rowmap(ny, -1);
+  int nb = 0;
+  for (int64_t y = 0; y < ny; ++y) {
+    const int64_t yp = y % n.M;
+    if (n.direct || yp < r || yp >= n.M - r) rowmap[y] = nb++;
+  }
+  n.nb = nb;
+  // output tiles: band rows over the full width, side columns elsewhere
+  std::vector<BandTile> tiles;
+  auto strip = [&](int64_t ya, int64_t yb, int x0, int width) {
+    for (int cx = 0; cx < width; cx += 64) {
+      const int nc = std::min(64, width - cx);
+      const int h = (64 / nc) * RB_BAND;
+      for (int64_t y0 = ya; y0 < yb; y0 += h)
+        tiles.push_back(BandTile{(int)y0, x0 + cx, (int)std::min<int64_t>(h, yb - y0), nc});
+    }
+  };
+  for (int64_t y = 0; y < ny;) {
+    const bool band = rowmap[y] >= 0;
+    int64_t e = y;
+    while (e < ny && (rowmap[e] >= 0) == band) ++e;
+    if (band) {
+      for (int x0 = 0; x0 < nx; x0 += 64) strip(y, e, x0, (int)std::min<int64_t>(64, nx - x0));
+    } else {
+      strip(y, e, 0, r);
+      strip(y, e, (int)nx - r, r);
+    }
+    y = e;
+  }
+  // kernel rows padded to a multiple of KU_BAND, staged in chunks of kc rows
+  std::vector<int> widths;
+  for (const BandTile& t : tiles)
+    if (std::find(widths.begin(), widths.end(), t.ncols) == widths.end()) widths.push_back(t.ncols);
+  auto lds_for = [&](int kc) {
+    size_t m = 0;
+    for (int nc : widths) m = std::max(m, band_lds(nc, r, kc));
+    return m;
+  };
+  n.kr = (w + KU_BAND - 1) / KU_BAND * KU_BAND;
+  n.kc = n.kr;
+  while (n.kc > KU_BAND && lds_for(n.kc) > 64 * 1024) n.kc -= KU_BAND;
+  n.lds = lds_for(n.kc);
+  if (n.lds > 150 * 1024) return fail(NLSE_ERR_ARG, "nonlocal kernel too wide (%d points)", kw);
+  std::vector<double> kp((size_t)n.kr * w, 0.0);
+  for (int u = 0; u < w; ++u)
+    for (int v = 0; v < w; ++v) kp[(size_t)u * w + v] = kernel[(size_t)(u + r0 - r) * kw + (v + r0 - r)];
+  int rc = upload((void**)&n.K, kp.data(), kp.size() * sizeof(double));
+  if (!rc) rc = upload((void**)&n.rowmap, rowmap.data(), rowmap.size() * sizeof(int));
+  if (!rc) rc = upload((void**)&n.tiles, tiles.data(), tiles.size() * sizeof(BandTile));
+  if (rc) return rc;
+  n.ntiles = (int)tiles.size();
+  HIP_TRY(hipMalloc((void**)&n.rowband, sizeof(float) * slabs * nb * nx));
+  if (!n.direct) {
+    HIP_TRY(hipMalloc((void**)&n.colband, sizeof(float) * slabs * ny * 2 * r));
+    HIP_TRY(hipMalloc((void**)&n.Z, sizeof(float2) * items * n.M * nx));
+    n.zops = line_ops(n.M);
+    if (!n.zops) return fail(NLSE_ERR_ARG, "unsupported intensity column length %lld", (long long)n.M);
+    if ((rc = n.zops->twiddles(&n.twz))) return rc;
+    n.colw = n.zops->colw(nx, p->d.col_width);
+    // spectrum of the kernel wrapped on the M x nx torus, centre at (0, 0):
+    // real and even, K^(ky, kx) = sum_{u,v} K[u][v] cos(2pi ky (u-r)/M) cos(2pi kx (v-r)/nx)
+    const int64_t hy = n.M / 2 + 1, hx = nx / 2 + 1;
+    std::vector<double> cx(nx), cy(n.M);
+    for (int64_t j = 0; j < nx; ++j) cx[j] = std::cos(2.0 * M_PI * (double)j / (double)nx);
+    for (int64_t j = 0; j < n.M; ++j) cy[j] = std::cos(2.0 * M_PI * (double)j / (double)n.M);
+    std::vector<double> row((size_t)w * hx);
+    for (int u = 0; u < w; ++u)
+      for (int64_t k = 0; k < hx; ++k) {
+        double acc = 0.0;
+        for (int v = 0; v < w; ++v) {
+          const int64_t ph = ((k * (v - r)) % nx + nx) % nx;
+          acc += kp[(size_t)u * w + v] * cx[ph];
+        }
+        row[(size_t)u * hx + k] = acc;
+      }
+    const double norm = 1.0 / ((double)n.M * (double)nx);
+    std::vector<float> kq((size_t)hy * hx);
+    for (int64_t ky = 0; ky < hy; ++ky)
+      for (int64_t k = 0; k < hx; ++k) {
+        double acc = 0.0;
+        for (int u = 0; u < w; ++u) {
+          const int64_t ph = ((ky * (u - r)) % n.M + n.M) % n.M;
+          acc += cy[ph] * row[(size_t)u * hx + k];
+        }
+        kq[(size_t)ky * hx + k] = (float)(acc * norm);
+      }
+    if ((rc = upload((void**)&n.kq, kq.data(), kq.size() * sizeof(float)))) return rc;
+  }
+  if (n.lds > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute((const void*)k_nl_band, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n.lds));
+  return NLSE_OK;
+}
+
+int nlse_plan_set_nonlocal(nlse_plan* p, const double* kernel, int64_t kh, int64_t kw) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  nlc_free(p);
+  if (!kernel) return NLSE_OK;
+  if (p->d.ndim == 3) return fail(NLSE_ERR_ARG, "in1 and in2 should have the same dimensionality");
+  if (kw < 1 || kh < 1 || kw % 2 == 0 || kh % 2 == 0)
+    return fail(NLSE_ERR_ARG, "nonlocal kernel sizes must be odd (got %lld x %lld)", (long long)kh, (long long)kw);
+  int rc;
+  if (p->d.ndim == 1) {
+    if (kh != 1) return fail(NLSE_ERR_ARG, "1-D plans take a 1-D nonlocal kernel");
+    const int r0 = (int)(kw / 2);
+    const int r = (int)std::min<int64_t>(r0, p->d.nx - 1);
+    rc = upload((void**)&p->nlc.K, kernel + (r0 - r), sizeof(double) * (2 * r + 1));
+    p->nlc.r = r;
+  } else {
+    if (kh != kw) return fail(NLSE_ERR_ARG, "2-D nonlocal kernels must be square");
+    if (kw > 8191) return fail(NLSE_ERR_ARG, "nonlocal kernel too wide (%lld points)", (long long)kw);
+    rc = nlc_setup_2d(p, kernel, (int)kw);
+  }
+  if (rc) {
+    nlc_free(p);
+    return rc;
+  }
+  p->nlc.on = true;
+  return NLSE_OK;
+}
+
 static int check_ready(const nlse_plan* p, int precision) {
   if (p->pkind < 0) return fail(NLSE_ERR_STATE, "propagator not set");
   if (precision == 0 && !p->nl[NLSE_SLOT_FULL]) return fail(NLSE_ERR_STATE, "nonlinear parameters (single) not set");
@@ -866,6 +858,123 @@ static RowArgs mid_row(const nlse_plan* p, const RowArgs& r, int precision, bool
   return a;
 }
 
+// ------------------------------------------- nonlocal response (2-D)
+static NlcArgs plan_nlc_args(const nlse_plan* p, void* A) {
+  NlcArgs a;
+  std::memset(&a, 0, sizeof a);
+  const auto& n = p->nlc;
+  a.A = (float2*)A;
+  a.Z = n.Z;
+  a.M = n.M;
+  a.ny = p->d.ny;
+  a.ncomp = p->d.ncomp;
+  a.ngroups = p->d.items * n.M;
+  a.tw = p->twx;
+  a.write_z = n.direct ? 0 : 1;
+  a.loss_sat = p->d.loss_saturates;
+  a.rabi = p->rabi;
+  a.V = p->V;
+  a.vkind = p->vkind;
+  a.v_item_stride = p->v_item_stride;
+  a.drv_r0 = p->fr0;
+  a.drv_r1 = p->fr1;
+  a.drv_dz = p->drv_dz;
+  a.out_scale = 1.f;
+  a.rowmap = n.rowmap;
+  a.rowband = n.rowband;
+  a.colband = n.colband;
+  a.nb = n.nb;
+  a.r = n.r;
+  a.direct = n.direct ? 1 : 0;
+  return a;
+}
+
+static int band_launch(const nlse_plan* p, const void* A, hipStream_t s) {
+  const auto& n = p->nlc;
+  const int64_t slabs = p->d.items * p->d.ncomp;
+  if (n.ntiles == 0) return NLSE_OK;
+  if (slabs > 65535) return fail(NLSE_ERR_ARG, "too many fields for the nonlocal band pass (%lld)", (long long)slabs);
+  BandArgs b;
+  b.A = (const float2*)A;
+  b.ny = (int)p->d.ny;
+  b.nx = (int)p->d.nx;
+  b.r = n.r;
+  b.K = n.K;
+  b.kr = n.kr;
+  b.kc = n.kc;
+  b.tiles = n.tiles;
+  b.rowmap = n.rowmap;
+  b.nb = n.nb;
+  b.rowband = n.rowband;
+  b.colband = n.colband;
+  hipLaunchKernelGGL(k_nl_band, dim3((unsigned)n.ntiles, (unsigned)slabs), dim3(64 * BAND_WAVES), n.lds, s, b);
+  return launch_status("k_nl_band");
+}
+
+static ColArgs nlc_col_args(const nlse_plan* p) {
+  ColArgs c = col_defaults();
+  c.A = p->nlc.Z;
+  c.nslabs = p->d.items;
+  c.NX = (int)p->d.nx;
+  c.ncomp = 1;
+  c.tw = p->nlc.twz;
+  c.do_fwd = c.do_prop = c.do_inv = 1;
+  c.pkind = PROP_KREAL;
+  c.kq = p->nlc.kq;
+  return c;
+}
+
+// One nonlinear application with the nonlocal response: [I_x,] the
+// convolved intensity, N, [Rabi, drive, F_x].  drive_step < 0: no drive.
+static int nlc_apply(nlse_plan* p, void* A, int inverse_first, int forward_last, const void* nlp, bool rabi,
+                     int64_t drive_step, hipStream_t s) {
+  NlcArgs a = plan_nlc_args(p, A);
+  a.inverse_first = inverse_first;
+  int rc = NLSE_OK;
+  if (a.inverse_first || a.write_z) rc = p->rops->nlc(1, a, s);
+  if (!rc) rc = band_launch(p, A, s);
+  if (!rc && !p->nlc.direct) rc = p->nlc.zops->col(p->nlc.colw, nlc_col_args(p), s);
+  if (rc) return rc;
+  a.forward_last = forward_last;
+  a.nlp = nlp;
+  a.has_rabi = rabi ? 1 : 0;
+  if (drive_step >= 0) {
+    a.has_drive = 1;
+    a.drv_t0 = p->ft0[drive_step];
+    a.drv_t1 = p->ft1[drive_step];
+  }
+  return p->rops->nlc(2, a, s);
+}
+
+// 2-D run with the nonlocal response.  Between steps the field is x-spectral
+// as in the local chain; each nonlinear application takes four launches.
+static int run2d_nonlocal(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, hipStream_t s) {
+  int rc;
+  if (precision == 1) {
+    rc = nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
+  } else {
+    RowArgs a = plan_row_args(p, A);
+    a.forward_last = 1;
+    rc = p->rops->row(p->d.ncomp, a, s);
+  }
+  if (rc) return rc;
+  const ColArgs c = plan_col_args(p, A);
+  const bool rabi = p->rabi != nullptr && p->d.ncomp == 2;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    const bool last = (k == nsteps - 1);
+    const int64_t ds = p->drive ? step0 + k : -1;
+    if ((rc = p->cops->col(p->colw, c, s))) return rc;
+    if (precision == 0) {
+      rc = nlc_apply(p, A, 1, last ? 0 : 1, p->nl[NLSE_SLOT_FULL], rabi, ds, s);
+    } else {
+      rc = nlc_apply(p, A, 1, 0, p->nl[NLSE_SLOT_HALF_POST], false, ds, s);
+      if (!rc && !last) rc = nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
+    }
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
 static int run1d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream_t s) {
   Line1DArgs a;
   std::memset(&a, 0, sizeof a);
@@ -883,6 +992,8 @@ static int run1d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream
   a.rabi = (precision == 0 && p->d.ncomp == 2) ? p->rabi : nullptr;
   a.V = p->V;
   a.vkind = p->vkind;
+  a.nlk = p->nlc.on ? p->nlc.K : nullptr;
+  a.nlr = p->nlc.r;
   return p->rops->line1d(p->d.ncomp, a, s);
 }
 
@@ -961,6 +1072,7 @@ int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0
     return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
                 (long long)(step0 + nsteps - 1));
   hipStream_t s = (hipStream_t)stream;
+  if (p->nlc.on) return run2d_nonlocal(p, A, nsteps, precision, step0, s);
   const int G = p->d.ncomp;
   const RowArgs r = plan_row_args(p, A);
   const ColArgs c = plan_col_args(p, A);
@@ -989,12 +1101,31 @@ int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0
 }
 
 int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision, void* stream) {
-  if (!p || !A || count < 0 || (pass != 0 && pass != 1) || (precision != 0 && precision != 1))
+  if (!p || !A || count < 0 || pass < 0 || pass > 5 || (precision != 0 && precision != 1))
     return fail(NLSE_ERR_ARG, "bad run_pass arguments");
   if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "run_pass times the sweeps of 2-D plans");
   int rc = check_ready(p, precision);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (pass >= 2) {
+    if (!p->nlc.on) return fail(NLSE_ERR_STATE, "passes 2-5 need a nonlocal response");
+    if (pass == 4 && p->nlc.direct) return fail(NLSE_ERR_STATE, "direct nonlocal plans have no intensity sweep");
+    const float inv_nx = (float)(1.0 / (double)p->d.nx);
+    NlcArgs a = plan_nlc_args(p, A);
+    a.out_scale = inv_nx;
+    a.inverse_first = 1;
+    a.forward_last = 1;
+    a.nlp = precision == 1 ? p->nl[NLSE_SLOT_HALF_POST] : p->nl[NLSE_SLOT_FULL];
+    ColArgs cz = nlc_col_args(p);
+    cz.scale = (float)p->d.nx;
+    for (int64_t k = 0; k < count && !rc; ++k) {
+      if (pass == 2) rc = p->rops->nlc(1, a, s);
+      else if (pass == 3) rc = band_launch(p, A, s);
+      else if (pass == 4) rc = p->nlc.zops->col(p->nlc.colw, cz, s);
+      else rc = p->rops->nlc(2, a, s);
+    }
+    return rc;
+  }
   // repeated sweeps of one kind: 1/N per sweep keeps the data at its scale
   // (in a run the 1/(NX NY) of the propagator normalises both transforms)
   RowArgs r = mid_row(p, plan_row_args(p, A), precision, false);

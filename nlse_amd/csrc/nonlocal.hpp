@@ -1,0 +1,341 @@
+// Nonlocal response of the split step (kernel built at nlse.py:143-153,
+// applied at nlse.py:406-409,444-447, cnlse.py:243-249,317-323): the
+// intensity |A|^2 is replaced by its zero-padded linear convolution with
+// the normalised K0 profile, mode "same", before the nonlinear operator.
+//
+// 2-D fields.  The bulk is a circular convolution computed by FFT on a
+// torus of M x NX points; the kernel is even in both axes, so its spectrum
+// is real and is stored as one quadrant.  Two real intensity fields share
+// one complex transform, Z = I_a + i I_b:
+//   one component   I_a = rows [0, NY/2), I_b = rows [NY/2, NY)  (M = NY/2)
+//   two components  I_a = |A1|^2, I_b = |A2|^2                   (M = NY)
+// Circular and zero-padded linear convolution agree except within r = 3d
+// points of the torus edges (rows y mod M in [0, r) or [M-r, M), columns in
+// [0, r) or [NX-r, NX)).  Those band points are convolved directly, in
+// FP64, by k_nl_band; when the bands would cover the torus (2r >= M or
+// 2r >= NX, or M < 8) every point is a band point ("direct" mode).
+//
+// One nonlinear application with the nonlocal response is four launches:
+//   k_nlc_z    [I_x of the field, stored in real space], Z, F_x(Z)
+//   k_nl_band  direct convolution at the band points
+//   k_col      (PROP_KREAL) F_y, x K^/(M NX), I_y of Z
+//   k_nlc_nl   I_x(Z), band substitution, nonlinear operator, [F_x]
+#pragma once
+#include "fft_line.hpp"
+#include "step_ops.hpp"
+
+namespace nlse {
+
+struct NlcArgs {
+  float2* A;
+  float2* Z;              // [items][M][NX] x-transformed packed intensities
+  int64_t ngroups;        // items * M line groups
+  int64_t M;              // torus rows
+  int64_t ny;             // NY
+  int ncomp;              // 1: row pairs (y, y+M); 2: component pairs
+  const float4* tw;       // row twiddles
+  // stage 1
+  int inverse_first;      // apply I_x to the field (then store it)
+  int write_z;            // build and transform Z (0 in direct mode)
+  // stage 2
+  int forward_last;       // apply F_x after the nonlinear operator
+  int loss_sat;
+  const void* nlp;        // NLParams1[items] or NLParams2[items*2]
+  int has_rabi;
+  int has_drive;
+  const RabiParams* rabi;
+  const void* V;
+  int vkind;
+  int64_t v_item_stride;
+  const float2* drv_r0;
+  const float2* drv_r1;
+  float2 drv_t0;
+  float2 drv_t1;
+  float drv_dz;
+  float out_scale;
+  // band values of the convolved intensity
+  const int* rowmap;      // [NY] band row index, or -1
+  const float* rowband;   // [slabs][nb][NX]
+  const float* colband;   // [slabs][NY][2r]
+  int nb;
+  int r;
+  int direct;
+};
+
+// Geometry of line group gi: the two field lines it owns and their slabs.
+struct NlcLines {
+  int64_t item, yp;
+  int64_t off[2];
+  int64_t slab[2];
+  int64_t y[2];
+};
+
+__device__ __forceinline__ NlcLines nlc_lines(const NlcArgs& a, int64_t gi, int n) {
+  NlcLines L;
+  L.item = gi / a.M;
+  L.yp = gi - L.item * a.M;
+  if (a.ncomp == 1) {
+    L.slab[0] = L.slab[1] = L.item;
+    L.y[0] = L.yp;
+    L.y[1] = L.yp + a.M;
+    L.off[0] = (L.item * a.ny + L.yp) * n;
+    L.off[1] = L.off[0] + a.M * n;
+  } else {
+    L.slab[0] = 2 * L.item;
+    L.slab[1] = 2 * L.item + 1;
+    L.y[0] = L.y[1] = L.yp;
+    L.off[0] = (L.slab[0] * a.ny + L.yp) * n;
+    L.off[1] = L.off[0] + a.ny * n;
+  }
+  return L;
+}
+
+// Stage 1: [I_x] of the two field lines (stored back in real space), then
+// Z = |A_a|^2 + i |A_b|^2 and its forward transform along x.
+template <int N, int P, int LPB>
+__global__ __launch_bounds__(LPB*(N / P)) void k_nlc_z(NlcArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  __shared__ float2 lds[LPB * LINE];
+  const int li = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t gi = (int64_t)blockIdx.x * LPB + li;
+  const bool active = gi < a.ngroups;
+  const NlcLines L = nlc_lines(a, active ? gi : 0, N);
+  float2* lline = lds + li * LINE;
+
+  float2 v[2][P];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[L.off[g] + t + m * T] : make_float2(0.f, 0.f);
+  if (a.inverse_first) {
+    fft_line<N, P, 2, true, 1>(v, t, a.tw, lline, LINE);
+    if (a.out_scale != 1.f) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].x * a.out_scale, v[g][m].y * a.out_scale);
+    }
+    if (active) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int m = 0; m < P; ++m) a.A[L.off[g] + t + m * T] = v[g][m];
+    }
+  }
+  if (a.write_z) {
+    float2 z[1][P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) z[0][m] = make_float2(sqmag(v[0][m]), sqmag(v[1][m]));
+    fft_line<N, P, 1, false>(z, t, a.tw, lline, LINE);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) a.Z[gi * N + t + m * T] = z[0][m];
+    }
+  }
+}
+
+// Convolved intensity of point x of a line in row y of slab `slab`: the
+// band value where (y, x) is a band point, else the circular value.
+__device__ __forceinline__ float nlc_band(const NlcArgs& a, int bi, int64_t slab, int64_t y, int x, int n,
+                                          float circ) {
+  if (bi >= 0) return a.rowband[(slab * a.nb + bi) * n + x];
+  if (x < a.r) return a.colband[(slab * a.ny + y) * 2 * a.r + x];
+  if (x >= n - a.r) return a.colband[(slab * a.ny + y) * 2 * a.r + (x - n + 2 * a.r)];
+  return circ;
+}
+
+// Stage 2: the nonlinear operator with the convolved intensities, then
+// [Rabi, drive, F_x].  The intensity transform runs first, while the field
+// loads are in flight.  One component (NC = 1): the two rows are finished
+// one after the other (potential, nonlinear step, F_x, store), so only one
+// row's working set is live at a time; two components (NC = 2) couple at
+// every point and are transformed one line after the other.
+template <int N, int P, int LPB, int NC>
+__global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  __shared__ float2 lds[LPB * LINE];
+  const int li = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t gi = (int64_t)blockIdx.x * LPB + li;
+  const bool active = gi < a.ngroups;
+  const int64_t gs = active ? gi : 0;
+  const NlcLines L = nlc_lines(a, gs, N);
+  float2* lline = lds + li * LINE;
+
+  float2 z[1][P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) z[0][m] = make_float2(0.f, 0.f);
+  if (!a.direct) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) z[0][m] = active ? a.Z[gs * N + t + m * T] : make_float2(0.f, 0.f);
+  }
+  float2 v[NC][P];
+#pragma unroll
+  for (int g = 0; g < NC; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[L.off[g] + t + m * T] : make_float2(0.f, 0.f);
+  if (!a.direct) fft_line<N, P, 1, true>(z, t, a.tw, lline, LINE);
+  // z.x: convolved intensity of line 0, z.y: of line 1
+  if (active) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int bi = a.rowmap[L.y[g]];
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const float circ = g == 0 ? z[0][m].x : z[0][m].y;
+        const float val = nlc_band(a, bi, L.slab[g], L.y[g], t + m * T, N, circ);
+        if (g == 0)
+          z[0][m].x = val;
+        else
+          z[0][m].y = val;
+      }
+    }
+  }
+
+  float vr[P], vi[P];
+  auto load_row_v = [&](int64_t y) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      vr[m] = 0.f;
+      vi[m] = 0.f;
+    }
+    if (a.vkind != V_NONE) {
+      const int64_t vbase = L.item * a.v_item_stride + y * N + t;
+#pragma unroll
+      for (int m = 0; m < P; ++m) load_v(a.V, a.vkind, vbase + m * T, vr[m], vi[m]);
+    }
+  };
+  auto finish_line = [&](float2 (&w)[1][P], int64_t off) {
+    if (a.forward_last) fft_line<N, P, 1, false>(w, t, a.tw, lline, LINE);
+    if (a.out_scale != 1.f) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) w[0][m] = make_float2(w[0][m].x * a.out_scale, w[0][m].y * a.out_scale);
+    }
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) a.A[off + t + m * T] = w[0][m];
+    }
+  };
+
+  if constexpr (NC == 1) {
+    const NLParams1 p = reinterpret_cast<const NLParams1*>(a.nlp)[L.item];
+    float I[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) I[m] = z[0][m].x;
+    load_row_v(L.y[0]);
+    nl1_block_I<P>(v[0], I, p, a.loss_sat, vr, vi);
+    finish_line(v, L.off[0]);
+    float2 w[1][P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      w[0][m] = active ? a.A[L.off[1] + t + m * T] : make_float2(0.f, 0.f);
+      I[m] = z[0][m].y;
+    }
+    load_row_v(L.y[1]);
+    nl1_block_I<P>(w[0], I, p, a.loss_sat, vr, vi);
+    finish_line(w, L.off[1]);
+  } else {
+    const NLParams2* pp = reinterpret_cast<const NLParams2*>(a.nlp) + 2 * L.item;
+    const NLParams2 p1 = pp[0], p2 = pp[1];
+    float I1[P], I2[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      I1[m] = z[0][m].x;
+      I2[m] = z[0][m].y;
+    }
+    load_row_v(L.yp);
+    nl2_block_I<P>(v[0], v[1], I1, I2, p1, p2, vr, vi);
+    if (a.has_rabi) {
+      const RabiParams rb = a.rabi[L.item];
+#pragma unroll
+      for (int m = 0; m < P; ++m) rabi(v[0][m], v[1][m], rb);
+    }
+    if (a.has_drive) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const int64_t ri = L.yp * N + t + m * T;
+        float2 q = cmul(a.drv_r0[ri], a.drv_t0);
+        q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+        v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+        q = cmul(a.drv_r1[ri], a.drv_t1);
+        q = make_float2(q.x * a.drv_dz, q.y * a.drv_dz);
+        v[1][m] = make_float2(v[1][m].x + q.y, v[1][m].y - q.x);
+      }
+    }
+    float2 w[1][P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) w[0][m] = v[0][m];
+    finish_line(w, L.off[0]);
+#pragma unroll
+    for (int m = 0; m < P; ++m) w[0][m] = v[1][m];
+    finish_line(w, L.off[1]);
+  }
+}
+
+// ------------------------------------------------------------ band points
+// Output rectangle of k_nl_band: rows [y0, y0 + nrows), columns
+// [x0, x0 + ncols), ncols <= 64.  Lane l of each wave owns column
+// x0 + l % ncols and the RB_BAND = 4 rows y0 + (l / ncols)*4 + j.
+struct BandTile {
+  int y0, x0, nrows, ncols;
+};
+
+constexpr int RB_BAND = 4;
+constexpr int BAND_WAVES = 8;
+constexpr int KU_BAND = 4;  // kernel rows per register window
+
+struct BandArgs {
+  const float2* A;        // real-space field, slabs of NY x NX
+  int ny, nx, r;
+  const double* K;        // [kr][2r+1] kernel, rows >= 2r+1 zero (kr % KU_BAND == 0)
+  int kr;                 // padded kernel rows
+  int kc;                 // kernel rows per LDS chunk (multiple of KU_BAND)
+  const BandTile* tiles;
+  const int* rowmap;
+  int nb;
+  float* rowband;
+  float* colband;
+};
+
+// LDS layout of one block: staged intensities [srows][wcols] (float), the
+// kernel chunk [kc][2r+1] (double), the per-wave partial sums
+// [BAND_WAVES][64][RB_BAND] (double).
+__host__ __device__ inline size_t band_img_floats(int nc, int r, int kc) {
+  const size_t srows = (size_t)(64 / nc) * RB_BAND + kc - 1;
+  return (srows * (size_t)(nc + 2 * r) + 1) & ~(size_t)1;
+}
+
+// ------------------------------------------------------------------ 1-D
+// Zero-padded linear convolution of the intensities of the G lines owned
+// by a thread group, through the line's LDS (N floats per line), FP64
+// accumulation: I[g][m] = sum_j K[x - j + r] |A_g(j)|^2, x = t + m*T.
+template <int N, int P, int G>
+__device__ __forceinline__ void conv1d_intensity(const float2 (&v)[G][P], float (&I)[G][P], float* ib, int t,
+                                                 const double* __restrict__ K, int r) {
+  constexpr int T = N / P;
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) ib[g * N + t + m * T] = sqmag(v[g][m]);
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int x = t + m * T;
+      const int j0 = x - r < 0 ? 0 : x - r;
+      const int j1 = x + r > N - 1 ? N - 1 : x + r;
+      double acc = 0.0;
+      for (int j = j0; j <= j1; ++j) acc = fma(K[x - j + r], (double)ib[g * N + j], acc);
+      I[g][m] = (float)acc;
+    }
+  __syncthreads();
+}
+
+}  // namespace nlse

@@ -14,11 +14,12 @@
 // real-space side inputs in the row pass.
 #pragma once
 #include "fft_line.hpp"
+#include "nonlocal.hpp"
 #include "step_ops.hpp"
 
 namespace nlse {
 
-enum : int { PROP_SEP64 = 0, PROP_C64 = 1, PROP_SEP3D = 2, PROP_C128 = 3 };
+enum : int { PROP_SEP64 = 0, PROP_C64 = 1, PROP_SEP3D = 2, PROP_C128 = 3, PROP_KREAL = 4 };
 
 struct RowArgs {
   float2* A;
@@ -163,6 +164,7 @@ struct ColArgs {
   const float2* ptab;  // [ncomp][NY][NX] dense complex64 propagator
   const double2* pt;   // [NT] temporal factor of the 3-D propagator (PROP_SEP3D)
   const double2* ptab128;  // [N][NX] dense complex128 propagator (PROP_C128)
+  const float* kq;     // [N/2+1][NX/2+1] real spectrum quadrant (PROP_KREAL)
   int nt;              // columns = NX * NT for PROP_SEP3D
   float scale;         // 1/(NX NY) folded into the propagator
   float out_scale;     // applied to the stored result when no propagator
@@ -232,6 +234,26 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep3d(v[g][m], px[g], py, pt[g], (double)a.scale);
       }
+    } else if (a.pkind == PROP_KREAL) {
+      // even real spectrum: entry (ky, kx) of the quadrant table at
+      // (min(ky, N-ky), min(kx, NX-kx))
+      const int hx = a.NX / 2 + 1;
+      int kx[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int col = c0 + g * WT;
+        kx[g] = col <= a.NX / 2 ? col : a.NX - col;
+      }
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const int ky = t + m * T;
+        const int kyq = ky <= N / 2 ? ky : N - ky;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const float f = a.kq[kyq * hx + kx[g]] * a.scale;
+          v[g][m] = make_float2(v[g][m].x * f, v[g][m].y * f);
+        }
+      }
     } else if (a.pkind == PROP_C128) {
       const double2* pt = a.ptab128 + c0;
 #pragma unroll
@@ -281,6 +303,8 @@ struct Line1DArgs {
   const RabiParams* rabi; // [items] or null
   const void* V;          // [N] raw potential
   int vkind;
+  const double* nlk;      // nonlocal kernel [2*nlr+1] (null: local response)
+  int nlr;
 };
 
 template <int N, int P, int LPB, int G>
@@ -318,7 +342,19 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_line1d(Line1DArgs a) {
     for (int m = 0; m < P; ++m) pr[g][m] = a.prop[g * N + t + m * T];
 
   auto apply_nl = [&](const void* prm) {
-    if constexpr (G == 1) {
+    if (a.nlk != nullptr) {
+      // nonlocal response: convolved intensities (nlse_1d.py:65-66 kernel)
+      float I[G][P];
+      conv1d_intensity<N, P, G>(v, I, reinterpret_cast<float*>(lline), t, a.nlk, a.nlr);
+      if constexpr (G == 1) {
+        const NLParams1 p = reinterpret_cast<const NLParams1*>(prm)[item];
+        nl1_block_I<P>(v[0], I[0], p, a.loss_sat, vr, vi);
+      } else {
+        const NLParams2* pp = reinterpret_cast<const NLParams2*>(prm) + 2 * item;
+        const NLParams2 p1 = pp[0], p2 = pp[1];
+        nl2_block_I<P>(v[0], v[1], I[0], I[1], p1, p2, vr, vi);
+      }
+    } else if constexpr (G == 1) {
       const NLParams1 p = reinterpret_cast<const NLParams1*>(prm)[item];
       nl1_block<P>(v[0], p, a.loss_sat, vr, vi);
     } else {

@@ -169,6 +169,39 @@ __device__ __forceinline__ void nl2_block(float2* A1, float2* A2, const struct N
   apply_phase_loss_block<P>(A2, a2, t2);
 }
 
+// nl_prop with a given intensity per point (the nonlocal response
+// replaces |A|^2 by its convolution with the kernel, nlse.py:444-447).
+template <int P>
+__device__ __forceinline__ void nl1_block_I(float2* A, const float* I, const struct NLParams1& p, bool loss_sat,
+                                            const float* vr, const float* vi) {
+  float a[P], th[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const float sat = recip(fmaf(I[m], p.inv_isat, 1.0f));
+    th[m] = fmaf(p.gdz * I[m], sat, p.vdz * vr[m]);
+    a[m] = fmaf(p.vdz, vi[m], loss_sat ? p.adz * sat : p.adz);
+  }
+  apply_phase_loss_block<P>(A, a, th);
+}
+
+template <int P>
+__device__ __forceinline__ void nl2_block_I(float2* A1, float2* A2, const float* I1, const float* I2,
+                                            const struct NLParams2& p1, const struct NLParams2& p2, const float* vr,
+                                            const float* vi) {
+  float a1[P], t1[P], a2[P], t2[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const float s1 = recip(fmaf(I2[m], p1.inv_is_o, fmaf(I1[m], p1.inv_is_s, 1.0f)));
+    const float s2 = recip(fmaf(I1[m], p2.inv_is_o, fmaf(I2[m], p2.inv_is_s, 1.0f)));
+    t1[m] = fmaf(p1.gs_dz * I1[m], s1, fmaf(p1.gc_dz * I2[m], s1, p1.vdz * vr[m]));
+    a1[m] = fmaf(p1.vdz, vi[m], p1.adz * s1);
+    t2[m] = fmaf(p2.gs_dz * I2[m], s2, fmaf(p2.gc_dz * I1[m], s2, p2.vdz * vr[m]));
+    a2[m] = fmaf(p2.vdz, vi[m], p2.adz * s2);
+  }
+  apply_phase_loss_block<P>(A1, a1, t1);
+  apply_phase_loss_block<P>(A2, a2, t2);
+}
+
 // V kinds
 enum : int { V_NONE = 0, V_REAL = 1, V_CPLX = 2 };
 

@@ -154,6 +154,26 @@ class Plan:
         _lib.check(lib.nlse_plan_set_drive(self._h, r0.data_ptr(), r1.data_ptr(), t0.ctypes.data, t1.ctypes.data,
                                            n, float(dz)), "set_drive")
 
+    def set_nonlocal(self, kernel) -> None:
+        """Nonlocal response kernel (host array: (k, k) for 2-D plans, (k,)
+        for 1-D plans), or None for the local response."""
+        lib = _lib.load()
+        if kernel is None:
+            if self._last.get("nonlocal") is not None:
+                _lib.check(lib.nlse_plan_set_nonlocal(self._h, None, 0, 0), "set_nonlocal")
+            self._last["nonlocal"] = None
+            return
+        k = np.ascontiguousarray(np.asarray(kernel, dtype=np.float64))
+        if k.ndim == 1:
+            k = k.reshape(1, -1)
+        if k.ndim != 2:
+            raise ValueError("nonlocal kernel must be 1-D or 2-D")
+        key = (k.shape, k.tobytes())
+        if self._last.get("nonlocal") == key:
+            return
+        _lib.check(lib.nlse_plan_set_nonlocal(self._h, k.ctypes.data, k.shape[0], k.shape[1]), "set_nonlocal")
+        self._last["nonlocal"] = key
+
     # -- execution -------------------------------------------------------
     def run(self, A: torch.Tensor, nsteps: int, precision: int = 0, step0: int = 0) -> None:
         if A.dtype != torch.complex64 or not A.is_contiguous() or not A.is_cuda:
@@ -165,7 +185,9 @@ class Plan:
 
     def run_pass(self, A: torch.Tensor, which: int, count: int, precision: int = 0) -> None:
         """Launch one sweep type `count` times (0: row sweep, 1: column
-        sweep) in place on A -- timing only, the data is overwritten."""
+        sweep; nonlocal response: 2 intensity transform, 3 band convolution,
+        4 intensity column sweep, 5 nonlinear step) in place on A -- timing
+        only, the data is overwritten."""
         _lib.check(_lib.load().nlse_run_pass(self._h, A.data_ptr(), int(which), int(count), int(precision),
                                              D.stream_ptr()), "run_pass")
 

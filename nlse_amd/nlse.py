@@ -318,8 +318,7 @@ class NLSE:
         return None
 
     def _configure(self, eng: Plan, A: torch.Tensor, V, propagator, precision: str) -> None:
-        if self.nl_length > 0:
-            self._configure_nonlocal(eng)
+        self._configure_nonlocal(eng)
         lead = self._param_lead(A)
         setter = eng.set_nl1 if self._ncomp() == 1 else eng.set_nl2
         for slot, rec in self._nl_slots(precision, lead, V).items():
@@ -330,7 +329,13 @@ class NLSE:
         self._configure_tables(eng, V, propagator)
 
     def _configure_nonlocal(self, eng: Plan) -> None:
-        raise NotImplementedError("nonlocal response is not available in this build of the engine")
+        """nlse.py:406-409,444-447: with nl_length > 0 every nonlinear
+        application sees oaconvolve(|A|^2, nl_profile, mode="same")."""
+        if self.nl_length > 0:
+            eng.set_nonlocal(D.to_host(self.nl_profile) if D.is_device_tensor(self.nl_profile)
+                             else np.asarray(self.nl_profile))
+        else:
+            eng.set_nonlocal(None)
 
     def _configure_tables(self, eng: Plan, V, propagator) -> None:
         pf = self._prop_factors

@@ -80,6 +80,13 @@ class NLSE_3d(NLSE):
         lead = self._param_lead(A)
         return dict(ndim=3, ny=self.NY, nx=self.NX, nt=self.NZ, items=int(np.prod(lead)) if lead else 1, ncomp=1)
 
+    def _configure_nonlocal(self, eng) -> None:
+        """The reference convolves the 3-D intensity with the 2-D nl_profile
+        over three axes (nlse.py:407-409), which scipy's oaconvolve rejects."""
+        if self.nl_length > 0:
+            raise ValueError("in1 and in2 should have the same dimensionality")
+        eng.set_nonlocal(None)
+
     def _configure_tables(self, eng, V, propagator) -> None:
         pf = self._prop_factors
         if pf is not None and pf[0] is propagator:
