@@ -194,6 +194,12 @@ int nlse_plan_set_nonlocal(nlse_plan* plan, const double* kernel, int64_t kh, in
  * index of the first step (drive time index). */
 int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t step0,
              void* stream);
+/* nlse_run for a field that stays at the same address: the steady steps
+ * are captured once into a hipGraph of `chunk` steps and replayed (the
+ * capture is kept until a table changes or the field moves).  Results are
+ * identical to nlse_run.  Plans with a drive profile and 1-D plans (one
+ * launch per run) run as nlse_run. */
+int nlse_run_graph(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t chunk, void* stream);
 /* Launch one sweep type `count` times in place on A (device): pass 0 is a
  * steady-state row sweep (I_x, nonlinear step, F_x), pass 1 a column sweep
  * (F_y, propagator, I_y); with a nonlocal response, passes 2-5 are its

@@ -88,6 +88,7 @@ SIGNATURES = {
     "nlse_plan_set_nonlocal": (c_int, [c_void_p, c_void_p, c_i64, c_i64]),
     "nlse_run": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
     "nlse_run_pass": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_int, c_void_p]),
+    "nlse_run_graph": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
     "nlse_plan_config": (c_int, [c_void_p, C.POINTER(c_int), C.POINTER(c_int)]),
 }
 

@@ -462,8 +462,11 @@ struct nlse_plan {
     int colw = 1;
     float2* Z = nullptr;            // [items][M][nx]
     float* kq = nullptr;            // [M/2+1][nx/2+1] spectrum / (M nx)
-    double* K = nullptr;            // 2-D: [kr][2r+1]; 1-D: [2r+1]
+    double* K = nullptr;            // 1-D kernel [2r+1]
+    float* KT = nullptr;            // 2-D kernel, transposed [2r+1][kr]
     int kr = 0, kc = 0;
+    int pitch = 0;                  // LDS row pitch of k_nl_band (0: tile width)
+
     int* rowmap = nullptr;          // [ny]
     int nb = 0;
     BandTile* tiles = nullptr;
@@ -472,13 +475,22 @@ struct nlse_plan {
     float* rowband = nullptr;
     float* colband = nullptr;
   } nlc;
+  // captured steady steps
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey;
 };
+
+static void graph_free(nlse_plan* p);
 
 static void nlc_free(nlse_plan* p) {
   auto& n = p->nlc;
   (void)hipFree(n.Z);
   (void)hipFree(n.kq);
   (void)hipFree(n.K);
+  (void)hipFree(n.KT);
+
   (void)hipFree(n.rowmap);
   (void)hipFree(n.tiles);
   (void)hipFree(n.rowband);
@@ -528,6 +540,8 @@ int nlse_plan_destroy(nlse_plan* p) {
   (void)hipFree(p->pt);
   (void)hipFree(p->rabi);
   nlc_free(p);
+  graph_free(p);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
   return NLSE_OK;
 }
@@ -548,6 +562,7 @@ static int upload(void** dst, const void* src, size_t bytes) {
 
 int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* py) {
   if (!p || !px || !py) return fail(NLSE_ERR_ARG, "null argument");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim == 1) return fail(NLSE_ERR_ARG, "1-D plans take nlse_plan_set_propagator_c128");
   const int nc = p->d.ncomp;
   int rc = upload((void**)&p->px, px, sizeof(double2) * nc * p->d.nx);
@@ -559,6 +574,7 @@ int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* p
 
 int nlse_plan_set_propagator_c128(nlse_plan* p, const double* prop) {
   if (!p || !prop) return fail(NLSE_ERR_ARG, "null argument");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 1) return fail(NLSE_ERR_ARG, "complex128 line propagator is for 1-D plans");
   int rc = upload((void**)&p->prop128, prop, sizeof(double2) * p->d.ncomp * p->d.nx);
   if (rc) return rc;
@@ -568,12 +584,14 @@ int nlse_plan_set_propagator_c128(nlse_plan* p, const double* prop) {
 
 int nlse_plan_set_propagator_t(nlse_plan* p, const double* pt) {
   if (!p || !pt) return fail(NLSE_ERR_ARG, "null argument");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "temporal propagator factor is for 3-D plans");
   return upload((void**)&p->pt, pt, sizeof(double2) * p->d.nt);
 }
 
 int nlse_plan_set_propagator_3d_c128(nlse_plan* p, const void* tab) {
   if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "dense complex128 propagators are for 3-D plans");
   p->ptab128 = (const double2*)tab;
   p->pkind = PROP_C128;
@@ -582,6 +600,7 @@ int nlse_plan_set_propagator_3d_c128(nlse_plan* p, const void* tab) {
 
 int nlse_plan_set_propagator_c64(nlse_plan* p, const void* tab) {
   if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "dense complex64 propagators are for 2-D plans");
   p->ptab = (const float2*)tab;
   p->pkind = PROP_C64;
@@ -592,6 +611,7 @@ static float inv_or_zero(double v) { return std::isinf(v) ? 0.f : (float)(1.0 / 
 
 int nlse_plan_set_nl1(nlse_plan* p, int slot, const nlse_nl1* prm) {
   if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl1 arguments");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ncomp != 1) return fail(NLSE_ERR_ARG, "set_nl1 on a two-component plan");
   std::vector<NLParams1> h(p->d.items);
   for (int64_t i = 0; i < p->d.items; ++i) {
@@ -605,6 +625,7 @@ int nlse_plan_set_nl1(nlse_plan* p, int slot, const nlse_nl1* prm) {
 
 int nlse_plan_set_nl2(nlse_plan* p, int slot, const nlse_nl2* prm) {
   if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl2 arguments");
+  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ncomp != 2) return fail(NLSE_ERR_ARG, "set_nl2 on a one-component plan");
   std::vector<NLParams2> h(p->d.items * 2);
   for (int64_t i = 0; i < p->d.items * 2; ++i) {
@@ -620,6 +641,7 @@ int nlse_plan_set_nl2(nlse_plan* p, int slot, const nlse_nl2* prm) {
 
 int nlse_plan_set_rabi(nlse_plan* p, const double* cm1, const double* sn) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  graph_free(p);  // captured launches bake in table pointers
   if (!cm1 || !sn) {
     (void)hipFree(p->rabi);
     p->rabi = nullptr;
@@ -635,6 +657,7 @@ int nlse_plan_set_rabi(nlse_plan* p, const double* cm1, const double* sn) {
 
 int nlse_plan_set_potential(nlse_plan* p, const void* V, int vkind, int64_t vstride) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  graph_free(p);  // captured launches bake in table pointers
   if (vkind != V_NONE && vkind != V_REAL && vkind != V_CPLX) return fail(NLSE_ERR_ARG, "bad v_kind %d", vkind);
   if (vkind != V_NONE && !V) return fail(NLSE_ERR_ARG, "null V");
   p->V = V;
@@ -646,6 +669,7 @@ int nlse_plan_set_potential(nlse_plan* p, const void* V, int vkind, int64_t vstr
 int nlse_plan_set_drive(nlse_plan* p, const void* fr0, const void* fr1, const float* ft0, const float* ft1,
                         int64_t n_t, double dz) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  graph_free(p);  // captured launches bake in table pointers
   if (!fr0) {
     p->drive = false;
     return NLSE_OK;
@@ -661,10 +685,7 @@ int nlse_plan_set_drive(nlse_plan* p, const void* fr0, const void* fr1, const fl
 }
 
 // LDS bytes of one k_nl_band block for tiles of `nc` columns.
-static size_t band_lds(int nc, int r, int kc) {
-  return band_img_floats(nc, r, kc) * sizeof(float) + (size_t)kc * (2 * r + 1) * sizeof(double) +
-         (size_t)BAND_WAVES * 64 * RB_BAND * sizeof(double);
-}
+static size_t band_lds(int nc, int r, int kc, int pitch) { return band_lds_floats(nc, r, kc, pitch) * sizeof(float); }
 
 // Band geometry and kernel spectrum of a 2-D plan (see nonlocal.hpp).
 static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
@@ -717,9 +738,14 @@ static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
   std::vector<int> widths;
   for (const BandTile& t : tiles)
     if (std::find(widths.begin(), widths.end(), t.ncols) == widths.end()) widths.push_back(t.ncols);
+  // LDS row pitch: fixed 128 or 256 floats (immediate offsets) when the
+  // widest staged tile fits, else the tile width
+  int wmax = 0;
+  for (int nc : widths) wmax = std::max(wmax, nc + 2 * r);
+  n.pitch = wmax <= 128 ? 128 : (wmax <= 256 ? 256 : 0);
   auto lds_for = [&](int kc) {
     size_t m = 0;
-    for (int nc : widths) m = std::max(m, band_lds(nc, r, kc));
+    for (int nc : widths) m = std::max(m, band_lds(nc, r, kc, n.pitch));
     return m;
   };
   n.kr = (w + KU_BAND - 1) / KU_BAND * KU_BAND;
@@ -730,7 +756,10 @@ static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
   std::vector<double> kp((size_t)n.kr * w, 0.0);
   for (int u = 0; u < w; ++u)
     for (int v = 0; v < w; ++v) kp[(size_t)u * w + v] = kernel[(size_t)(u + r0 - r) * kw + (v + r0 - r)];
-  int rc = upload((void**)&n.K, kp.data(), kp.size() * sizeof(double));
+  std::vector<float> kt((size_t)w * n.kr, 0.f);
+  for (int u = 0; u < w; ++u)
+    for (int v = 0; v < w; ++v) kt[(size_t)v * n.kr + u] = (float)kp[(size_t)u * w + v];
+  int rc = upload((void**)&n.KT, kt.data(), kt.size() * sizeof(float));
   if (!rc) rc = upload((void**)&n.rowmap, rowmap.data(), rowmap.size() * sizeof(int));
   if (!rc) rc = upload((void**)&n.tiles, tiles.data(), tiles.size() * sizeof(BandTile));
   if (rc) return rc;
@@ -772,13 +801,17 @@ static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
       }
     if ((rc = upload((void**)&n.kq, kq.data(), kq.size() * sizeof(float)))) return rc;
   }
-  if (n.lds > 64 * 1024)
-    HIP_TRY(hipFuncSetAttribute((const void*)k_nl_band, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n.lds));
+  if (n.lds > 64 * 1024) {
+    const void* fn = n.pitch == 128 ? (const void*)k_nl_band<128>
+                                    : (n.pitch == 256 ? (const void*)k_nl_band<256> : (const void*)k_nl_band<0>);
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n.lds));
+  }
   return NLSE_OK;
 }
 
 int nlse_plan_set_nonlocal(nlse_plan* p, const double* kernel, int64_t kh, int64_t kw) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  graph_free(p);  // captured launches bake in table pointers
   nlc_free(p);
   if (!kernel) return NLSE_OK;
   if (p->d.ndim == 3) return fail(NLSE_ERR_ARG, "in1 and in2 should have the same dimensionality");
@@ -899,7 +932,7 @@ static int band_launch(const nlse_plan* p, const void* A, hipStream_t s) {
   b.ny = (int)p->d.ny;
   b.nx = (int)p->d.nx;
   b.r = n.r;
-  b.K = n.K;
+  b.KT = n.KT;
   b.kr = n.kr;
   b.kc = n.kc;
   b.tiles = n.tiles;
@@ -907,7 +940,13 @@ static int band_launch(const nlse_plan* p, const void* A, hipStream_t s) {
   b.nb = n.nb;
   b.rowband = n.rowband;
   b.colband = n.colband;
-  hipLaunchKernelGGL(k_nl_band, dim3((unsigned)n.ntiles, (unsigned)slabs), dim3(64 * BAND_WAVES), n.lds, s, b);
+  const dim3 grid((unsigned)n.ntiles, (unsigned)slabs), block(64 * BAND_WAVES);
+  if (n.pitch == 128)
+    hipLaunchKernelGGL(k_nl_band<128>, grid, block, n.lds, s, b);
+  else if (n.pitch == 256)
+    hipLaunchKernelGGL(k_nl_band<256>, grid, block, n.lds, s, b);
+  else
+    hipLaunchKernelGGL(k_nl_band<0>, grid, block, n.lds, s, b);
   return launch_status("k_nl_band");
 }
 
@@ -931,10 +970,13 @@ static int nlc_apply(nlse_plan* p, void* A, int inverse_first, int forward_last,
   NlcArgs a = plan_nlc_args(p, A);
   a.inverse_first = inverse_first;
   int rc = NLSE_OK;
+  auto& n = p->nlc;
   if (a.inverse_first || a.write_z) rc = p->rops->nlc(1, a, s);
-  if (!rc) rc = band_launch(p, A, s);
-  if (!rc && !p->nlc.direct) rc = p->nlc.zops->col(p->nlc.colw, nlc_col_args(p), s);
   if (rc) return rc;
+  // one stream: running the band pass on a second stream beside the
+  // intensity sweep costs more in cross-queue waits than it overlaps
+  if ((rc = band_launch(p, A, s))) return rc;
+  if (!n.direct && (rc = n.zops->col(n.colw, nlc_col_args(p), s))) return rc;
   a.forward_last = forward_last;
   a.nlp = nlp;
   a.has_rabi = rabi ? 1 : 0;
@@ -944,35 +986,6 @@ static int nlc_apply(nlse_plan* p, void* A, int inverse_first, int forward_last,
     a.drv_t1 = p->ft1[drive_step];
   }
   return p->rops->nlc(2, a, s);
-}
-
-// 2-D run with the nonlocal response.  Between steps the field is x-spectral
-// as in the local chain; each nonlinear application takes four launches.
-static int run2d_nonlocal(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, hipStream_t s) {
-  int rc;
-  if (precision == 1) {
-    rc = nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
-  } else {
-    RowArgs a = plan_row_args(p, A);
-    a.forward_last = 1;
-    rc = p->rops->row(p->d.ncomp, a, s);
-  }
-  if (rc) return rc;
-  const ColArgs c = plan_col_args(p, A);
-  const bool rabi = p->rabi != nullptr && p->d.ncomp == 2;
-  for (int64_t k = 0; k < nsteps; ++k) {
-    const bool last = (k == nsteps - 1);
-    const int64_t ds = p->drive ? step0 + k : -1;
-    if ((rc = p->cops->col(p->colw, c, s))) return rc;
-    if (precision == 0) {
-      rc = nlc_apply(p, A, 1, last ? 0 : 1, p->nl[NLSE_SLOT_FULL], rabi, ds, s);
-    } else {
-      rc = nlc_apply(p, A, 1, 0, p->nl[NLSE_SLOT_HALF_POST], false, ds, s);
-      if (!rc && !last) rc = nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
-    }
-    if (rc) return rc;
-  }
-  return NLSE_OK;
 }
 
 static int run1d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream_t s) {
@@ -997,30 +1010,51 @@ static int run1d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream
   return p->rops->line1d(p->d.ncomp, a, s);
 }
 
+// ---------------------------------------------------------- step chains
+// Every multi-sweep run is first(), then step(k) for k = 0..nsteps-1, where
+// step(k) ends with the sweep that returns the field to real space when k
+// is the last step.  Steps that are not the last are "steady": the same
+// launches with the same arguments (drive aside), which is what a captured
+// graph replays.
+//
+// 2-D (local): between sweeps the field is x-spectral, B = F_x(A).
+//   first:  row pass F_x [after N(dz/2) in "double"]
+//   step:   column pass (F_y, P/(NX NY), I_y), row pass (I_x, N, [Rabi,
+//           drive,] [N(dz/2) of the next step,] F_x unless last)
+// 2-D (nonlocal): the row pass becomes one or two nonlocal applications
+//   (nlc_apply: intensity transform, band pass, intensity sweep, N).
 // 3-D field (items, NY, NX, NT), t fastest.  Between sweeps the field is
-// t- and x-spectral; one step is four sweeps:
+// t- and x-spectral; a step is four sweeps:
 //   y columns: F_y, P/(NX NY NT), I_y   x columns: I_x
-//   t rows:    I_t, N, F_t               x columns: F_x
-static int run3d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream_t s) {
-  const int64_t ny = p->d.ny, nx = p->d.nx, nt = p->d.nt, items = p->d.items;
+//   t rows:    I_t, N, F_t               x columns: F_x (unless last)
+
+static RowArgs row3d_args(const nlse_plan* p, void* A) {
   RowArgs r = row_defaults();
   r.A = (float2*)A;
-  r.rows_per_item = ny * nx;
-  r.ngroups = items * ny * nx;
+  r.rows_per_item = p->d.ny * p->d.nx;
+  r.ngroups = p->d.items * p->d.ny * p->d.nx;
   r.tw = p->twt;
   r.loss_sat = p->d.loss_saturates;
   r.V = p->V;
   r.vkind = p->vkind;
   r.v_item_stride = p->v_item_stride;
+  return r;
+}
+
+static ColArgs colx3d_args(const nlse_plan* p, void* A) {
   ColArgs cx = col_defaults();
   cx.A = (float2*)A;
-  cx.nslabs = items * ny;
-  cx.NX = (int)nt;
+  cx.nslabs = p->d.items * p->d.ny;
+  cx.NX = (int)p->d.nt;
   cx.tw = p->twx;
+  return cx;
+}
+
+static ColArgs coly3d_args(const nlse_plan* p, void* A) {
   ColArgs cy = col_defaults();
   cy.A = (float2*)A;
-  cy.nslabs = items;
-  cy.NX = (int)(nx * nt);
+  cy.nslabs = p->d.items;
+  cy.NX = (int)(p->d.nx * p->d.nt);
   cy.tw = p->twy;
   cy.do_fwd = cy.do_prop = cy.do_inv = 1;
   cy.pkind = p->pkind;
@@ -1028,75 +1062,146 @@ static int run3d(nlse_plan* p, void* A, int64_t nsteps, int precision, hipStream
   cy.px = p->px;
   cy.py = p->py;
   cy.pt = p->pt;
-  cy.nt = (int)nt;
-  cy.scale = (float)(1.0 / ((double)nx * (double)ny * (double)nt));
+  cy.nt = (int)p->d.nt;
+  cy.scale = (float)(1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nt));
+  return cy;
+}
+
+static int chain_first(nlse_plan* p, void* A, int precision, hipStream_t s) {
   int rc;
-  {
-    RowArgs a = r;
+  if (p->d.ndim == 3) {
+    RowArgs a = row3d_args(p, A);
     a.forward_last = 1;
     a.has_b = precision == 1;
     a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
     if ((rc = p->tops->row(1, a, s))) return rc;
-    ColArgs c = cx;
+    ColArgs c = colx3d_args(p, A);
     c.do_fwd = 1;
-    if ((rc = p->rops->col(p->colw_x, c, s))) return rc;
+    return p->rops->col(p->colw_x, c, s);
   }
-  for (int64_t k = 0; k < nsteps; ++k) {
-    const bool last = (k == nsteps - 1);
-    if ((rc = p->cops->col(p->colw, cy, s))) return rc;
-    ColArgs ci = cx;
+  if (p->nlc.on && precision == 1) return nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
+  RowArgs a = plan_row_args(p, A);
+  a.forward_last = 1;
+  a.has_b = (!p->nlc.on && precision == 1) ? 1 : 0;
+  a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
+  return p->rops->row(p->d.ncomp, a, s);
+}
+
+// Step k of a run (absolute step index `abs_k` for the drive profile).
+static int chain_step(nlse_plan* p, void* A, int precision, bool last, int64_t abs_k, hipStream_t s) {
+  int rc;
+  if (p->d.ndim == 3) {
+    if ((rc = p->cops->col(p->colw, coly3d_args(p, A), s))) return rc;
+    ColArgs ci = colx3d_args(p, A);
     ci.do_inv = 1;
     if ((rc = p->rops->col(p->colw_x, ci, s))) return rc;
-    RowArgs a = mid_row(p, r, precision, last);
+    RowArgs a = mid_row(p, row3d_args(p, A), precision, last);
     if ((rc = p->tops->row(1, a, s))) return rc;
-    if (!last) {
-      ColArgs cf = cx;
-      cf.do_fwd = 1;
-      if ((rc = p->rops->col(p->colw_x, cf, s))) return rc;
-    }
+    if (last) return NLSE_OK;
+    ColArgs cf = colx3d_args(p, A);
+    cf.do_fwd = 1;
+    return p->rops->col(p->colw_x, cf, s);
   }
+  if ((rc = p->cops->col(p->colw, plan_col_args(p, A), s))) return rc;
+  if (p->nlc.on) {
+    const int64_t ds = p->drive ? abs_k : -1;
+    if (precision == 0) {
+      const bool rabi = p->rabi != nullptr && p->d.ncomp == 2;
+      return nlc_apply(p, A, 1, last ? 0 : 1, p->nl[NLSE_SLOT_FULL], rabi, ds, s);
+    }
+    if ((rc = nlc_apply(p, A, 1, 0, p->nl[NLSE_SLOT_HALF_POST], false, ds, s))) return rc;
+    return last ? NLSE_OK : nlc_apply(p, A, 0, 1, p->nl[NLSE_SLOT_HALF_PRE], false, -1, s);
+  }
+  RowArgs a = mid_row(p, plan_row_args(p, A), precision, last);
+  a.has_drive = p->drive ? 1 : 0;
+  if (p->drive) {
+    a.drv_t0 = p->ft0[abs_k];
+    a.drv_t1 = p->ft1[abs_k];
+  }
+  return p->rops->row(p->d.ncomp, a, s);
+}
+
+static int run_checks(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0) {
+  if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
+  int rc = check_ready(p, precision);
+  if (rc) return rc;
+  if (p->d.ndim == 3 && p->pkind == PROP_SEP3D && !p->pt)
+    return fail(NLSE_ERR_STATE, "3-D temporal propagator factor not set");
+  if (p->drive && p->d.ndim == 2 && step0 + nsteps > (int64_t)p->ft0.size())
+    return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
+                (long long)(step0 + nsteps - 1));
   return NLSE_OK;
 }
 
 int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0, void* stream) {
-  if (!p || !A || nsteps < 0 || (precision != 0 && precision != 1)) return fail(NLSE_ERR_ARG, "bad run arguments");
-  if (nsteps == 0) return NLSE_OK;
-  int rc = check_ready(p, precision);
-  if (rc) return rc;
-  if (p->d.ndim == 1) return run1d(p, A, nsteps, precision, (hipStream_t)stream);
-  if (p->d.ndim == 3) {
-    if (p->pkind == PROP_SEP3D && !p->pt) return fail(NLSE_ERR_STATE, "3-D temporal propagator factor not set");
-    return run3d(p, A, nsteps, precision, (hipStream_t)stream);
-  }
-  if (p->drive && step0 + nsteps > (int64_t)p->ft0.size())
-    return fail(NLSE_ERR_ARG, "drive time profile too short (%zu for step %lld)", p->ft0.size(),
-                (long long)(step0 + nsteps - 1));
+  int rc = run_checks(p, A, nsteps, precision, step0);
+  if (rc || nsteps == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (p->nlc.on) return run2d_nonlocal(p, A, nsteps, precision, step0, s);
-  const int G = p->d.ncomp;
-  const RowArgs r = plan_row_args(p, A);
-  const ColArgs c = plan_col_args(p, A);
-  // first row pass: F_x, preceded by the half step of "double"
-  {
-    RowArgs a = r;
-    a.forward_last = 1;
-    a.has_b = precision == 1;
-    a.nlp_b = p->nl[NLSE_SLOT_HALF_PRE];
-    rc = p->rops->row(G, a, s);
-    if (rc) return rc;
+  if (p->d.ndim == 1) return run1d(p, A, nsteps, precision, s);
+  if ((rc = chain_first(p, A, precision, s))) return rc;
+  for (int64_t k = 0; k < nsteps; ++k)
+    if ((rc = chain_step(p, A, precision, k == nsteps - 1, step0 + k, s))) return rc;
+  return NLSE_OK;
+}
+
+// ------------------------------------------------------- graph replay
+// The steady steps of a run are captured once per (field, precision,
+// chunk, plan tables) into a hipGraph of `chunk` steps and replayed; the
+// first sweep, the last step and the remainder are launched directly.  The
+// key holds every pointer and flag the captured launches bake in, so a
+// change of tables re-captures.
+static std::vector<uint64_t> graph_key(const nlse_plan* p, const void* A, int precision, int64_t chunk) {
+  std::vector<uint64_t> k = {(uint64_t)(uintptr_t)A, (uint64_t)precision, (uint64_t)chunk, (uint64_t)p->pkind,
+                             (uint64_t)(uintptr_t)p->px, (uint64_t)(uintptr_t)p->py,
+                             (uint64_t)(uintptr_t)p->ptab, (uint64_t)(uintptr_t)p->ptab128,
+                             (uint64_t)(uintptr_t)p->pt, (uint64_t)(uintptr_t)p->V, (uint64_t)p->vkind,
+                             (uint64_t)p->v_item_stride, (uint64_t)(uintptr_t)p->rabi,
+                             (uint64_t)(uintptr_t)p->nlc.KT, (uint64_t)(uintptr_t)p->nlc.Z, (uint64_t)p->nlc.on};
+  for (void* q : p->nl) k.push_back((uint64_t)(uintptr_t)q);
+  return k;
+}
+
+static void graph_free(nlse_plan* p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  p->gkey.clear();
+}
+
+static int graph_capture(nlse_plan* p, void* A, int precision, int64_t chunk) {
+  graph_free(p);
+  if (!p->cap) HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+  HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+  int rc = NLSE_OK;
+  for (int64_t k = 0; k < chunk && !rc; ++k) rc = chain_step(p, A, precision, false, 0, p->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->cap, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
   }
-  for (int64_t k = 0; k < nsteps; ++k) {
-    rc = p->cops->col(p->colw, c, s);
-    if (rc) return rc;
-    RowArgs a = mid_row(p, r, precision, k == nsteps - 1);
-    a.has_drive = p->drive ? 1 : 0;
-    if (p->drive) {
-      a.drv_t0 = p->ft0[step0 + k];
-      a.drv_t1 = p->ft1[step0 + k];
-    }
-    rc = p->rops->row(G, a, s);
-    if (rc) return rc;
-  }
+  if (e != hipSuccess) return fail(NLSE_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  p->graph = g;
+  HIP_TRY(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+  p->gkey = graph_key(p, A, precision, chunk);
+  return NLSE_OK;
+}
+
+int nlse_run_graph(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t chunk, void* stream) {
+  int rc = run_checks(p, A, nsteps, precision, 0);
+  if (rc || nsteps == 0) return rc;
+  if (chunk < 1) return fail(NLSE_ERR_ARG, "graph chunk must be >= 1");
+  if (p->d.ndim == 1 || p->drive) return nlse_run(p, A, nsteps, precision, 0, stream);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t steady = nsteps - 1;
+  const int64_t reps = steady / chunk;
+  if (reps > 0 && p->gkey != graph_key(p, A, precision, chunk) && (rc = graph_capture(p, A, precision, chunk)))
+    return rc;
+  if ((rc = chain_first(p, A, precision, s))) return rc;
+  for (int64_t r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(p->gexec, s));
+  for (int64_t k = reps * chunk; k < nsteps; ++k)
+    if ((rc = chain_step(p, A, precision, k == nsteps - 1, k, s))) return rc;
   return NLSE_OK;
 }
 

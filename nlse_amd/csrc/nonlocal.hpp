@@ -280,20 +280,20 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
 // ------------------------------------------------------------ band points
 // Output rectangle of k_nl_band: rows [y0, y0 + nrows), columns
 // [x0, x0 + ncols), ncols <= 64.  Lane l of each wave owns column
-// x0 + l % ncols and the RB_BAND = 4 rows y0 + (l / ncols)*4 + j.
+// x0 + l % ncols and the RB_BAND rows y0 + (l / ncols)*RB_BAND + j.
 struct BandTile {
   int y0, x0, nrows, ncols;
 };
 
-constexpr int RB_BAND = 4;
-constexpr int BAND_WAVES = 8;
-constexpr int KU_BAND = 4;  // kernel rows per register window
+constexpr int RB_BAND = 4;     // output rows per lane
+constexpr int BAND_WAVES = 4;  // waves per block, splitting the kernel columns
+constexpr int KU_BAND = 4;     // kernel rows per register window
 
 struct BandArgs {
   const float2* A;        // real-space field, slabs of NY x NX
   int ny, nx, r;
-  const double* K;        // [kr][2r+1] kernel, rows >= 2r+1 zero (kr % KU_BAND == 0)
-  int kr;                 // padded kernel rows
+  const float* KT;        // [2r+1][kr] transposed kernel, entries u >= 2r+1 zero
+  int kr;                 // padded kernel rows (multiple of KU_BAND)
   int kc;                 // kernel rows per LDS chunk (multiple of KU_BAND)
   const BandTile* tiles;
   const int* rowmap;
@@ -302,12 +302,15 @@ struct BandArgs {
   float* colband;
 };
 
-// LDS layout of one block: staged intensities [srows][wcols] (float), the
-// kernel chunk [kc][2r+1] (double), the per-wave partial sums
-// [BAND_WAVES][64][RB_BAND] (double).
-__host__ __device__ inline size_t band_img_floats(int nc, int r, int kc) {
+// LDS of one k_nl_band block, in floats: staged intensities
+// [groups*RB_BAND + kc - 1][pitch], the kernel chunk [2r+1][kc] and the
+// per-wave partial sums [BAND_WAVES][64][RB_BAND] (double).
+__host__ __device__ inline size_t band_lds_floats(int nc, int r, int kc, int pitch) {
   const size_t srows = (size_t)(64 / nc) * RB_BAND + kc - 1;
-  return (srows * (size_t)(nc + 2 * r) + 1) & ~(size_t)1;
+  const size_t p = pitch > 0 ? (size_t)pitch : (size_t)(nc + 2 * r);
+  const size_t img = (srows * p + 3) & ~(size_t)3;
+  const size_t kt = ((size_t)(2 * r + 1) * kc + 1) & ~(size_t)1;
+  return img + kt + (size_t)BAND_WAVES * 64 * RB_BAND * 2;
 }
 
 // ------------------------------------------------------------------ 1-D

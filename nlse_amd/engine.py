@@ -41,6 +41,8 @@ class Plan:
         self._h = h
         self._keep = {}
         self._last = {}
+        # steady steps per captured hipGraph (0: launch every step directly)
+        self.graph_chunk = 16
 
     # -- tables ----------------------------------------------------------
     def set_propagator_sep(self, px: np.ndarray, py: np.ndarray) -> None:
@@ -180,8 +182,13 @@ class Plan:
             raise TypeError("field must be a contiguous complex64 device tensor")
         if A.numel() != self.items * self.ncomp * self.ny * self.nx * self.nt:
             raise ValueError("field size does not match the plan")
-        _lib.check(_lib.load().nlse_run(self._h, A.data_ptr(), int(nsteps), int(precision), int(step0),
-                                        D.stream_ptr()), "run")
+        lib = _lib.load()
+        if self.graph_chunk > 0 and step0 == 0 and self.ndim != 1 and nsteps > self.graph_chunk:
+            _lib.check(lib.nlse_run_graph(self._h, A.data_ptr(), int(nsteps), int(precision),
+                                          int(self.graph_chunk), D.stream_ptr()), "run_graph")
+        else:
+            _lib.check(lib.nlse_run(self._h, A.data_ptr(), int(nsteps), int(precision), int(step0),
+                                    D.stream_ptr()), "run")
 
     def run_pass(self, A: torch.Tensor, which: int, count: int, precision: int = 0) -> None:
         """Launch one sweep type `count` times (0: row sweep, 1: column
