@@ -116,8 +116,10 @@ def kernel_bytes(P: int, N: int, r: int, local: bool) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", choices=("c1", "c2", "c3"), default="c1",
+                    help="c1: 2048^2 nonlocal split step (the metric); c2: uneven batch; c3: 3-D with GVD")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--local", action="store_true", help="drop the nonlocal term (config-1 physics otherwise)")
     ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
@@ -128,7 +130,17 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every GPU box here has one "
               "GPU; the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
         return 2
+    defaults = {"c1": (1000, 50), "c2": (500, 20), "c3": (200, 10)}[args.workload]
+    args.steps = defaults[0] if args.steps is None else args.steps
+    args.warmup = defaults[1] if args.warmup is None else args.warmup
+    if args.workload == "c2":
+        return bench_c2(args)
+    if args.workload == "c3":
+        return bench_c3(args)
+    return bench_c1(args)
 
+
+def bench_c1(args) -> int:
     import torch
 
     from nlse_amd import NLSE
@@ -233,6 +245,160 @@ def main() -> int:
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads, nl_length)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _timed(fn, *a) -> float:
+    """Seconds of one call of fn, bracketed by synchronisation and HIP
+    events on the current stream."""
+    import torch
+
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    fn(*a)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e-3
+
+
+def bench_c2(args) -> int:
+    """Uneven batch: 64 fields of 256^2/512^2/1024^2 with random n2, power,
+    Isat, one persistent launch per sweep for the whole batch, steady steps
+    replayed from a captured graph.  Compared with the same fields run one
+    after the other through their own plans."""
+    import torch
+
+    from nlse_amd import NLSE, NLSEBatch
+    from nlse_amd import workloads as W
+
+    sims, fields = W.config2(NLSE)
+    batch = NLSEBatch(sims)
+    eng, buf, _ = batch.prepare(fields, 1e-5)
+    P = eng.total
+    eng.run(buf, args.warmup)
+    t = _timed(eng.run, buf, args.steps)
+    sps = args.steps / t
+    finite = bool(torch.isfinite(torch.view_as_real(buf)).all())
+    kern = {}
+    for name, which, nbytes in (("row", 0, P * 16), ("col", 1, P * 16)):
+        eng.run_pass(buf, which, 5)
+        avg = _timed(eng.run_pass, buf, which, args.steps) / args.steps
+        kern[name] = {"avg_us": avg * 1e6, "bytes": nbytes, "GBps": nbytes / avg / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    achieved = kern[dom]["bytes"] / (kern[dom]["avg_us"] * 1e-6)
+    # the same fields through separate plans, one after the other
+    seq = []
+    for s, E in zip(sims, fields):
+        s._own_propagator()
+        s._send_arrays_to_gpu()
+        A, _ = s._prepare_output_array(E, True)
+        e = s._engine_for(A)
+        s._configure(e, A, s.V, s.propagator, "single")
+        seq.append((e, A))
+    nseq = max(10, args.steps // 10)
+    for e, A in seq:
+        e.run(A, 2)
+    t_seq = _timed(lambda: [e.run(A, nseq) for e, A in seq]) / nseq
+    result = {
+        "metric": "batch split-steps/s (64 uneven fields, every field one step)",
+        "value": sps,
+        "unit": "batch-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "config-2: 64 NLSE fields, N uniform on {256, 512, 1024} (rng 2), n2 = -10^U(-10,-8), "
+                               "P ~ U(0.1, 2) W, Isat ~ U(1e4, 1e6), alpha=20, V=None, dz=1e-5, precision='single'",
+                   "points": P, "batch": eng.config(), "graph_chunk": eng.graph_chunk},
+        "roofline": {"bound": "hbm", "kernel": f"batch {dom} sweep", "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "kernels": kern,
+        "step_roofline": {"B_per_point": 32, "achieved_GBps": sps * P * 32 / 1e9, "frac": sps * P * 32 / HBM_PEAK},
+        "grid_point_steps_per_s": sps * P,
+        "separate_plans_steps_per_s": 1.0 / t_seq,
+        "batch_speedup_vs_separate": sps * t_seq,
+        "field_finite": finite,
+    }
+    if not args.no_cpu:
+        import oracle
+
+        threads = min(os.cpu_count() or 1, 16)
+        osims, ofields = W.config2(oracle.NLSE, fft="pocketfft32", workers=threads, threads=threads)
+        t0, n, pts = time.perf_counter(), 0, 0
+        for s, E in zip(osims, ofields):
+            A, A_sq = s._prepare_output_array(E, True)
+            prop = s._build_propagator()
+            s.split_step(A, A_sq, None, prop, "single")
+            n += 1
+            pts += E.size
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": pts / el / P, "unit": "batch-steps/s", "cores": threads, "kind": "port",
+            "sample": f"one split step of each of the first {n} config-2 fields on the CPU oracle (complex64, "
+                      f"scipy.fft with {threads} workers), {pts} grid points in {el:.1f} s, scaled to the batch's "
+                      f"{P} points; host CPU: {cpu_model()}"}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_c3(args) -> int:
+    """3-D (x, y, t) NLSE 512x512x64 with GVD: four sweeps per step."""
+    import torch
+
+    from nlse_amd import NLSE_3d
+    from nlse_amd import workloads as W
+
+    sim = W.config3(NLSE_3d)
+    E0 = W.config3_field(sim)
+    sim._own_propagator()
+    sim._send_arrays_to_gpu()
+    A, _ = sim._prepare_output_array(E0, True)
+    eng = sim._engine_for(A)
+    sim._configure(eng, A, sim.V, sim.propagator, "single")
+    P = A.numel()
+    cell = sim.delta_X * sim.delta_Y * sim.delta_T
+
+    def energy():
+        return float(torch.view_as_real(A).double().pow(2).sum()) * cell * 299792458.0 * 8.8541878128e-12 / 2
+
+    e_start = energy()
+    eng.run(A, args.warmup)
+    t = _timed(eng.run, A, args.steps)
+    e_end = energy()
+    sps = args.steps / t
+    nbytes = P * 64  # four sweeps of 16 B/pt (V = None)
+    result = {
+        "metric": "split-steps/s, 3-D NLSE 512x512x64 c64",
+        "value": sps,
+        "unit": "split-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "config-3: NLSE_3d 512x512x64, D0=-1e-26, window [4e-3, 4e-3, 2e-12], n2=-1e-9, "
+                               "alpha=0, Isat=inf, dz=1e-5, Gaussian x sech(t/200fs), precision='single'",
+                   "grid": list(A.shape)},
+        "roofline": {"bound": "hbm", "kernel": "step (4 sweeps)", "achieved": sps * nbytes / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": sps * nbytes / HBM_PEAK, "traffic": None},
+        "step_roofline": {"B_per_point": 64, "min_B_per_point": 48, "frac_of_min_schedule": sps * P * 48 / HBM_PEAK},
+        "grid_point_steps_per_s": sps * P,
+        "energy_drift_per_step": (e_end - e_start) / e_start / (args.warmup + args.steps),
+    }
     print(json.dumps(result), flush=True)
     return 0
 

@@ -210,6 +210,36 @@ int nlse_run_pass(nlse_plan* plan, void* A_dev, int pass, int64_t count, int pre
 /* Kernel configuration actually used (rows per block, columns per block). */
 int nlse_plan_config(const nlse_plan* plan, int* row_lines, int* col_width);
 
+/* ------------------------------------------------------------------
+ * Uneven batches (no counterpart in the reference: its batching is the
+ * leading-dimension broadcast of one grid, nlse.py:320-327).  Many 2-D
+ * single-component NLSE fields of different grids and parameters advance
+ * together: one persistent launch per sweep for the whole batch, steady
+ * steps replayed from a captured hipGraph.  Grids: powers of two in
+ * [128, 2048] on both axes. */
+typedef struct nlse_batch nlse_batch;
+typedef struct {
+  int64_t ny, nx;
+  int64_t offset;     /* first complex64 element of this field in the batch buffer */
+  const double* px;   /* host complex128 [nx]: exp(-i kx^2 dz / 2k) */
+  const double* py;   /* host complex128 [ny] */
+  nlse_nl1 nl[3];     /* slots NLSE_SLOT_FULL, _HALF_PRE, _HALF_POST */
+  const void* V_dev;  /* device float32 / complex64 [ny][nx], or NULL */
+  int v_kind;
+} nlse_batch_item;
+int nlse_batch_create(const nlse_batch_item* items, int64_t count, int loss_saturates, nlse_batch** out);
+int nlse_batch_destroy(nlse_batch* batch);
+/* Work units and persistent grid sizes of the row and column sweeps. */
+int nlse_batch_config(const nlse_batch* batch, int64_t* row_units, int64_t* col_units, int* row_grid,
+                      int* col_grid);
+/* nsteps split steps of every field of the batch buffer `base_dev` (real
+ * space in and out); chunk > 0 replays chunks of steady steps from a
+ * captured graph (results identical to chunk = 0). */
+int nlse_batch_run(nlse_batch* batch, void* base_dev, int64_t nsteps, int precision, int64_t chunk,
+                   void* stream);
+/* Timing only: `count` row (pass 0) or column (pass 1) sweeps of the batch. */
+int nlse_batch_run_pass(nlse_batch* batch, void* base_dev, int pass, int64_t count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

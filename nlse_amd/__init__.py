@@ -16,3 +16,4 @@ from .gpe import GPE  # noqa: F401
 from .nlse import NLSE  # noqa: F401
 from .nlse_1d import NLSE_1d  # noqa: F401
 from .nlse_3d import NLSE_3d  # noqa: F401
+from .batch import NLSEBatch  # noqa: F401,E402

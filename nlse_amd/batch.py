@@ -1,0 +1,172 @@
+"""Uneven batches: many 2-D NLSE instances of different grids and physical
+parameters advanced together by the engine (nlse_batch_* in nlse_hip.h).
+
+The reference batches only through leading-dimension broadcasting of one
+grid (nlse.py:320-327); instances of different sizes are separate
+out_field calls.  Here every instance keeps its own NLSE object (grid,
+alpha, n2, I_sat, power, V, delta_z), and NLSEBatch.out_field propagates
+all of them with one persistent launch per sweep for the whole batch:
+
+    sims = [NLSE(alpha, P_i, window, n2_i, None, L, NX=N_i, NY=N_i, Isat=I_i) ...]
+    outs = NLSEBatch(sims).out_field([E_i ...], z)
+
+Each field follows exactly the step sequence of its own out_field (same
+kernels, same arithmetic), so the results equal separate runs.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE
+from .nlse import NLSE, count_steps
+
+_SUPPORTED = (128, 256, 512, 1024, 2048)
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("ny", C.c_int64), ("nx", C.c_int64), ("offset", C.c_int64), ("px", C.c_void_p),
+                ("py", C.c_void_p), ("nl", _lib.NL1 * 3), ("V_dev", C.c_void_p), ("v_kind", C.c_int)]
+
+
+class BatchEngine:
+    """Handle of one nlse_batch (tables and work lists of a fixed set of
+    fields).  `fields` is one contiguous complex64 device buffer."""
+
+    def __init__(self, items: list[dict], loss_saturates: bool = False):
+        arr = (BatchItem * len(items))()
+        self._keep = []
+        self.offsets, self.shapes = [], []
+        off = 0
+        for i, it in enumerate(items):
+            ny, nx = it["shape"]
+            px = np.ascontiguousarray(np.asarray(it["px"], np.complex128).reshape(nx))
+            py = np.ascontiguousarray(np.asarray(it["py"], np.complex128).reshape(ny))
+            self._keep += [px, py]
+            rec = arr[i]
+            rec.ny, rec.nx, rec.offset = ny, nx, off
+            rec.px, rec.py = px.ctypes.data, py.ctypes.data
+            for slot in range(3):
+                r = it["nl"][slot]
+                rec.nl[slot] = _lib.NL1(float(r["adz"]), float(r["gdz"]), float(r["inv_isat"]), float(r["vdz"]))
+            V = it.get("V")
+            if V is None:
+                rec.V_dev, rec.v_kind = None, 0
+            else:
+                V = V.contiguous()
+                if V.numel() != ny * nx:
+                    raise ValueError(f"batch field {i}: V does not match the grid")
+                self._keep.append(V)
+                rec.V_dev, rec.v_kind = V.data_ptr(), (1 if V.dtype == torch.float32 else 2)
+            self.offsets.append(off)
+            self.shapes.append((ny, nx))
+            off += ny * nx
+        self.total = off
+        h = C.c_void_p()
+        _lib.check(_lib.load().nlse_batch_create(arr, len(items), int(bool(loss_saturates)), C.byref(h)),
+                   "batch_create")
+        self._h = h
+        self.graph_chunk = 16
+
+    def config(self) -> dict:
+        a, b = C.c_int64(), C.c_int64()
+        c, d = C.c_int(), C.c_int()
+        _lib.check(_lib.load().nlse_batch_config(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),
+                   "batch_config")
+        return {"row_units": a.value, "col_units": b.value, "row_grid": c.value, "col_grid": d.value}
+
+    def run(self, fields: torch.Tensor, nsteps: int, precision: int = 0) -> None:
+        if fields.dtype != torch.complex64 or not fields.is_cuda or not fields.is_contiguous():
+            raise TypeError("batch fields must be one contiguous complex64 device buffer")
+        if fields.numel() != self.total:
+            raise ValueError(f"batch buffer has {fields.numel()} points, expected {self.total}")
+        _lib.check(_lib.load().nlse_batch_run(self._h, fields.data_ptr(), int(nsteps), int(precision),
+                                              int(self.graph_chunk), D.stream_ptr()), "batch_run")
+
+    def run_pass(self, fields: torch.Tensor, which: int, count: int) -> None:
+        """Timing only: `count` row (0) or column (1) sweeps in place."""
+        _lib.check(_lib.load().nlse_batch_run_pass(self._h, fields.data_ptr(), int(which), int(count),
+                                                   D.stream_ptr()), "batch_run_pass")
+
+    def views(self, fields: torch.Tensor) -> list[torch.Tensor]:
+        return [fields[o:o + ny * nx].view(ny, nx) for o, (ny, nx) in zip(self.offsets, self.shapes)]
+
+    def __del__(self):  # noqa: D105
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.nlse_batch_destroy(h)
+            self._h = None
+
+
+class NLSEBatch:
+    """A list of 2-D NLSE (or GPE) instances propagated together."""
+
+    def __init__(self, sims: list[NLSE]):
+        if not sims:
+            raise ValueError("empty batch")
+        for s in sims:
+            if s._ncomp() != 1 or len(s._last_axes) != 2:
+                raise ValueError("batches hold 2-D single-component instances (NLSE, GPE)")
+            if s.nl_length > 0:
+                raise NotImplementedError("batched instances use the local response (nl_length = 0)")
+            if s.NX not in _SUPPORTED or s.NY not in _SUPPORTED:
+                raise ValueError(f"batch grids are powers of two in [128, 2048] (got {s.NY}x{s.NX})")
+        if len({bool(s.loss_saturates) for s in sims}) != 1:
+            raise ValueError("all instances of a batch share the loss semantics")
+        self.sims = list(sims)
+        self.graph_chunk = 16
+
+    def _items(self, precision: str, z: float) -> list[dict]:
+        items = []
+        for s in self.sims:
+            if s.propagator is None:
+                s._own_propagator()
+            if s._prop_factors is None or s._prop_factors[0] is not s.propagator:
+                raise ValueError("batched instances use their own (separable) propagator")
+            n2_old = s.n2
+            if z > s.L:
+                s.n2 = 0  # nlse.py:571-572
+            slots = s._nl_slots(precision, ())
+            s.n2 = n2_old
+            zero = np.zeros(1, NL1_DTYPE)
+            nl = [slots.get(k, zero)[0] for k in (SLOT_FULL, SLOT_HALF_PRE, SLOT_HALF_POST)]
+            V = None if s.V is None else D.to_device(s.V)
+            if V is not None and V.dtype == torch.float64:
+                V = V.to(torch.float32)
+            if V is not None and V.dtype == torch.complex128:
+                V = V.to(torch.complex64)
+            items.append({"shape": (s.NY, s.NX), "px": s._prop_factors[1], "py": s._prop_factors[2],
+                          "nl": nl, "V": V})
+        return items
+
+    def prepare(self, E_list, z: float, precision: str = "single", normalize: bool = True):
+        """Engine and filled batch buffer for out_field (also used to time
+        the batch without host transfers)."""
+        if len(E_list) != len(self.sims):
+            raise ValueError("one input field per instance")
+        nsteps = {count_steps(z, s.delta_z) for s in self.sims}
+        if len(nsteps) != 1:
+            raise ValueError("all instances of a batch take the same number of steps")
+        eng = BatchEngine(self._items(precision, z), self.sims[0].loss_saturates)
+        eng.graph_chunk = self.graph_chunk
+        buf = torch.empty(eng.total, dtype=torch.complex64, device="cuda")
+        for s, E, v in zip(self.sims, E_list, eng.views(buf)):
+            assert tuple(E.shape) == (s.NY, s.NX), "Shape mismatch"
+            A, _ = s._prepare_output_array(E, normalize)
+            v.copy_(A)
+        return eng, buf, nsteps.pop()
+
+    def out_field(self, E_list, z: float, precision: str = "single", normalize: bool = True) -> list:
+        """Propagate every field over z; returns the fields in the input's
+        kind (numpy in, numpy out)."""
+        flag = self.sims[0]._precision_flag(precision)
+        eng, buf, nsteps = self.prepare(E_list, z, precision, normalize)
+        eng.run(buf, nsteps, flag)
+        outs = [v.clone() for v in eng.views(buf)]
+        torch.cuda.synchronize()
+        return [o.cpu().numpy() if isinstance(E, np.ndarray) else o for o, E in zip(outs, E_list)]

@@ -1,0 +1,392 @@
+// Uneven batches of 2-D single-component fields (nlse_batch_*): many NLSE
+// instances of different grid sizes and physical parameters advanced
+// together, one persistent launch per sweep and line length for the whole
+// batch.
+//
+// Work units are uniform in size whatever the grid: a row unit is
+// 4096/NX rows of NX points (256 threads, 16 points each) and a column
+// unit is 8192/NY columns of NY points (512 threads).  Every unit therefore
+// moves the same bytes and does the same FFT work per point, and a
+// persistent grid that strides over the unit list is balanced without a
+// work queue.  Each block reads its unit's instance descriptor (field
+// pointer, tables, parameters) and runs the sweep body of that line length.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/nlse_hip.h"
+#include "common.hpp"
+#include "lines.hpp"
+
+namespace nlse {
+
+struct BatchInst {
+  RowArgs row;
+  ColArgs col;
+  const void* nl[3];  // NLParams1 of the three slots
+  int nx, ny;
+  float inv_nx, inv_ny;
+};
+
+struct BatchUnit {
+  int inst;
+  int pad;
+  int64_t first;  // first row (row units) or column tile (column units)
+};
+
+struct BatchFlags {
+  int inverse_first, forward_last, has_a, has_b, slot_a, slot_b;
+  int timing;  // repeated sweeps: rescale by 1/N per sweep
+};
+
+constexpr int BROW_THREADS = 256;
+constexpr int BCOL_THREADS = 512;
+
+template <int N>
+struct BRow {
+  static constexpr int T = N / 16;
+  static constexpr int LPB = BROW_THREADS / T;
+  static constexpr int LDS = row_lds_size<N, LPB, 1>();
+};
+
+template <int N>
+struct BCol {
+  static constexpr int T = N / 16;
+  static constexpr int WT = BCOL_THREADS / T;
+  static constexpr int LDS = col_lds_size<N, WT, 1>();
+};
+
+static bool batch_len_ok(int64_t n) { return n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
+static int len_index(int64_t n) { return n == 128 ? 0 : n == 256 ? 1 : n == 512 ? 2 : n == 1024 ? 3 : 4; }
+constexpr int NLEN = 5;
+
+// One persistent kernel per line length: a single length per kernel keeps
+// the register footprint of the plain sweeps (a kernel holding every
+// length spills).
+template <int N>
+__global__ __launch_bounds__(BROW_THREADS) void k_batch_row(const BatchInst* __restrict__ inst,
+                                                            const BatchUnit* __restrict__ units, int64_t nunits,
+                                                            BatchFlags f) {
+  __shared__ float2 lds[BRow<N>::LDS];
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const BatchUnit un = units[u];
+    const BatchInst& I = inst[un.inst];
+    RowArgs a = I.row;
+    a.inverse_first = f.inverse_first;
+    a.forward_last = f.forward_last;
+    a.has_a = f.has_a;
+    a.has_b = f.has_b;
+    a.nlp_a = I.nl[f.slot_a];
+    a.nlp_b = I.nl[f.slot_b];
+    a.out_scale = f.timing ? I.inv_nx : 1.f;
+    row_body<N, 16, BRow<N>::LPB, 1>(a, un.first, lds);
+    __syncthreads();
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(BCOL_THREADS) void k_batch_col(const BatchInst* __restrict__ inst,
+                                                            const BatchUnit* __restrict__ units, int64_t nunits,
+                                                            BatchFlags f) {
+  __shared__ float2 lds[BCol<N>::LDS];
+  // XCD-aware order: consecutive list entries (neighbouring column tiles,
+  // which share 128-byte lines when narrow) go to blocks on the same XCD
+  const int64_t grid = gridDim.x;
+  const int64_t bb = (grid % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (grid / 8) + blockIdx.x / 8 : blockIdx.x;
+  for (int64_t u = bb; u < nunits; u += grid) {
+    const BatchUnit un = units[u];
+    const BatchInst& I = inst[un.inst];
+    ColArgs c = I.col;
+    if (f.timing) c.scale = I.inv_ny;
+    col_body<N, 16, BCol<N>::WT, 1, 1, PROP_SEP64>(c, un.first, lds);
+    __syncthreads();
+  }
+}
+
+}  // namespace nlse
+
+using namespace nlse;
+
+struct nlse_batch {
+  int64_t count = 0;
+  std::vector<BatchInst> h_inst;
+  std::vector<int64_t> offsets;
+  BatchInst* d_inst = nullptr;
+  // units grouped by line length: rows by NX, columns by NY
+  BatchUnit* d_row = nullptr;
+  BatchUnit* d_col = nullptr;
+  int64_t n_row = 0, n_col = 0;
+  int64_t row_off[NLEN + 1] = {0}, col_off[NLEN + 1] = {0};
+  int row_grid[NLEN] = {0}, col_grid[NLEN] = {0};
+  std::vector<void*> owned;  // device tables
+  const void* base = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gprec = -1;
+  int64_t gchunk = 0;
+};
+
+static int batch_alloc(nlse_batch* b, void** dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMalloc(dst, bytes));
+  b->owned.push_back(*dst);
+  HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return NLSE_OK;
+}
+
+static void batch_graph_free(nlse_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gprec = -1;
+}
+
+static BatchFlags row_flags(int precision, int stage) {
+  // stage 0: first sweep, 1: steady, 2: last
+  BatchFlags f;
+  std::memset(&f, 0, sizeof f);
+  f.inverse_first = stage > 0;
+  f.forward_last = stage < 2;
+  f.has_a = stage > 0;
+  f.slot_a = precision == 1 ? NLSE_SLOT_HALF_POST : NLSE_SLOT_FULL;
+  f.has_b = (precision == 1 && stage < 2) ? 1 : 0;
+  f.slot_b = NLSE_SLOT_HALF_PRE;
+  return f;
+}
+
+template <int N>
+static void launch_row_n(nlse_batch* b, int li, const BatchFlags& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_row<N>, dim3((unsigned)b->row_grid[li]), dim3(BROW_THREADS), 0, s, b->d_inst,
+                     b->d_row + b->row_off[li], b->row_off[li + 1] - b->row_off[li], f);
+}
+
+template <int N>
+static void launch_col_n(nlse_batch* b, int li, const BatchFlags& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_col<N>, dim3((unsigned)b->col_grid[li]), dim3(BCOL_THREADS), 0, s, b->d_inst,
+                     b->d_col + b->col_off[li], b->col_off[li + 1] - b->col_off[li], f);
+}
+
+// One persistent launch per line length present in the batch.
+static int batch_row(nlse_batch* b, const BatchFlags& f, hipStream_t s) {
+  for (int li = 0; li < NLEN; ++li) {
+    if (b->row_off[li + 1] == b->row_off[li]) continue;
+    switch (li) {
+      case 0: launch_row_n<128>(b, li, f, s); break;
+      case 1: launch_row_n<256>(b, li, f, s); break;
+      case 2: launch_row_n<512>(b, li, f, s); break;
+      case 3: launch_row_n<1024>(b, li, f, s); break;
+      default: launch_row_n<2048>(b, li, f, s); break;
+    }
+    int rc = launch_status("k_batch_row");
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
+static int batch_col(nlse_batch* b, hipStream_t s, int timing = 0) {
+  BatchFlags f;
+  std::memset(&f, 0, sizeof f);
+  f.timing = timing;
+  for (int li = 0; li < NLEN; ++li) {
+    if (b->col_off[li + 1] == b->col_off[li]) continue;
+    switch (li) {
+      case 0: launch_col_n<128>(b, li, f, s); break;
+      case 1: launch_col_n<256>(b, li, f, s); break;
+      case 2: launch_col_n<512>(b, li, f, s); break;
+      case 3: launch_col_n<1024>(b, li, f, s); break;
+      default: launch_col_n<2048>(b, li, f, s); break;
+    }
+    int rc = launch_status("k_batch_col");
+    if (rc) return rc;
+  }
+  return NLSE_OK;
+}
+
+static int batch_steady(nlse_batch* b, int precision, hipStream_t s) {
+  int rc = batch_col(b, s);
+  return rc ? rc : batch_row(b, row_flags(precision, 1), s);
+}
+
+static int batch_bind(nlse_batch* b, void* base, hipStream_t s) {
+  if (base == b->base) return NLSE_OK;
+  // launches already queued may still read the table
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < b->count; ++i) {
+    float2* A = (float2*)base + b->offsets[i];
+    b->h_inst[i].row.A = A;
+    b->h_inst[i].col.A = A;
+  }
+  HIP_TRY(hipMemcpy(b->d_inst, b->h_inst.data(), b->h_inst.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+  b->base = base;
+  return NLSE_OK;
+}
+
+extern "C" {
+
+int nlse_batch_create(const nlse_batch_item* items, int64_t count, int loss_saturates, nlse_batch** out) {
+  if (!items || !out || count < 1) return fail(NLSE_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  for (int64_t i = 0; i < count; ++i) {
+    const nlse_batch_item& it = items[i];
+    if (!batch_len_ok(it.ny) || !batch_len_ok(it.nx))
+      return fail(NLSE_ERR_ARG, "batch field %lld: grid (%lld, %lld) unsupported (128..2048, powers of two)",
+                  (long long)i, (long long)it.ny, (long long)it.nx);
+    if (!it.px || !it.py || it.offset < 0) return fail(NLSE_ERR_ARG, "batch field %lld: bad tables", (long long)i);
+    if (it.v_kind != V_NONE && (!it.V_dev || (it.v_kind != V_REAL && it.v_kind != V_CPLX)))
+      return fail(NLSE_ERR_ARG, "batch field %lld: bad potential", (long long)i);
+  }
+  nlse_batch* b = new nlse_batch();
+  b->count = count;
+  int rc = NLSE_OK;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  std::vector<BatchUnit> rows_n[NLEN], cols_n[NLEN];
+  for (int64_t i = 0; i < count && !rc; ++i) {
+    const nlse_batch_item& it = items[i];
+    BatchInst I;
+    std::memset(&I, 0, sizeof I);
+    I.nx = (int)it.nx;
+    I.ny = (int)it.ny;
+    I.inv_nx = (float)(1.0 / (double)it.nx);
+    I.inv_ny = (float)(1.0 / (double)it.ny);
+    const float4 *twx = nullptr, *twy = nullptr;
+    const LineOps* ox = line_ops_public(it.nx);
+    const LineOps* oy = line_ops_public(it.ny);
+    if ((rc = ox->twiddles(&twx)) || (rc = oy->twiddles(&twy))) break;
+    void *px = nullptr, *py = nullptr, *nl = nullptr;
+    if ((rc = batch_alloc(b, &px, it.px, sizeof(double2) * it.nx))) break;
+    if ((rc = batch_alloc(b, &py, it.py, sizeof(double2) * it.ny))) break;
+    NLParams1 h[3];
+    for (int sl = 0; sl < 3; ++sl) {
+      h[sl].adz = (float)it.nl[sl].adz;
+      h[sl].gdz = (float)it.nl[sl].gdz;
+      h[sl].inv_isat = std::isinf(it.nl[sl].inv_isat) ? 0.f : (float)it.nl[sl].inv_isat;
+      h[sl].vdz = (float)it.nl[sl].vdz;
+    }
+    if ((rc = batch_alloc(b, &nl, h, sizeof h))) break;
+    for (int sl = 0; sl < 3; ++sl) I.nl[sl] = (const NLParams1*)nl + sl;
+    RowArgs& r = I.row;
+    r.ngroups = it.ny;
+    r.rows_per_item = it.ny;
+    r.tw = twx;
+    r.loss_sat = loss_saturates;
+    r.V = it.V_dev;
+    r.vkind = it.v_kind;
+    r.out_scale = 1.f;
+    ColArgs& c = I.col;
+    c.nslabs = 1;
+    c.NX = (int)it.nx;
+    c.ncomp = 1;
+    c.tw = twy;
+    c.do_fwd = c.do_prop = c.do_inv = 1;
+    c.pkind = PROP_SEP64;
+    c.px = (const double2*)px;
+    c.py = (const double2*)py;
+    c.scale = (float)(1.0 / ((double)it.nx * (double)it.ny));
+    c.out_scale = 1.f;
+    b->h_inst.push_back(I);
+    b->offsets.push_back(it.offset);
+    const int64_t lpb = 4096 / it.nx, w = 8192 / it.ny;
+    for (int64_t y = 0; y < it.ny; y += lpb) rows_n[len_index(it.nx)].push_back(BatchUnit{(int)i, 0, y});
+    for (int64_t t = 0; t < it.nx / w; ++t) cols_n[len_index(it.ny)].push_back(BatchUnit{(int)i, 0, t});
+  }
+  std::vector<BatchUnit> rows, cols;
+  for (int li = 0; li < NLEN; ++li) {
+    b->row_off[li] = (int64_t)rows.size();
+    b->col_off[li] = (int64_t)cols.size();
+    rows.insert(rows.end(), rows_n[li].begin(), rows_n[li].end());
+    cols.insert(cols.end(), cols_n[li].begin(), cols_n[li].end());
+    // persistent grids: a few resident blocks per CU (column blocks hold
+    // about 74 KB of LDS, so two fit a CU)
+    b->row_grid[li] = (int)std::min<int64_t>((int64_t)rows_n[li].size(), (int64_t)ncu * 4);
+    b->col_grid[li] = (int)std::min<int64_t>((int64_t)cols_n[li].size(), (int64_t)ncu * 2);
+  }
+  b->row_off[NLEN] = (int64_t)rows.size();
+  b->col_off[NLEN] = (int64_t)cols.size();
+  if (!rc) rc = batch_alloc(b, (void**)&b->d_inst, b->h_inst.data(), b->h_inst.size() * sizeof(BatchInst));
+  if (!rc) rc = batch_alloc(b, (void**)&b->d_row, rows.data(), rows.size() * sizeof(BatchUnit));
+  if (!rc) rc = batch_alloc(b, (void**)&b->d_col, cols.data(), cols.size() * sizeof(BatchUnit));
+  if (rc) {
+    nlse_batch_destroy(b);
+    return rc;
+  }
+  b->n_row = (int64_t)rows.size();
+  b->n_col = (int64_t)cols.size();
+  *out = b;
+  return NLSE_OK;
+}
+
+int nlse_batch_destroy(nlse_batch* b) {
+  if (!b) return NLSE_OK;
+  batch_graph_free(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (void* q : b->owned) (void)hipFree(q);
+  delete b;
+  return NLSE_OK;
+}
+
+int nlse_batch_config(const nlse_batch* b, int64_t* row_units, int64_t* col_units, int* row_grid, int* col_grid) {
+  if (!b) return fail(NLSE_ERR_ARG, "null batch");
+  if (row_units) *row_units = b->n_row;
+  if (col_units) *col_units = b->n_col;
+  int rg = 0, cg = 0;
+  for (int li = 0; li < NLEN; ++li) {
+    rg += b->row_grid[li];
+    cg += b->col_grid[li];
+  }
+  if (row_grid) *row_grid = rg;  // blocks over the launches of one sweep
+  if (col_grid) *col_grid = cg;
+  return NLSE_OK;
+}
+
+int nlse_batch_run(nlse_batch* b, void* base, int64_t nsteps, int precision, int64_t chunk, void* stream) {
+  if (!b || !base || nsteps < 0 || (precision != 0 && precision != 1) || chunk < 0)
+    return fail(NLSE_ERR_ARG, "bad batch run arguments");
+  if (nsteps == 0) return NLSE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = batch_bind(b, base, s);
+  if (rc) return rc;
+  const int64_t reps = chunk > 0 ? (nsteps - 1) / chunk : 0;
+  if (reps > 0 && (b->gprec != precision || b->gchunk != chunk)) {
+    // capture `chunk` steady steps; the kernels read the instance table,
+    // so a new field buffer only rewrites the table
+    batch_graph_free(b);
+    if (!b->cap) HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    for (int64_t k = 0; k < chunk && !rc; ++k) rc = batch_steady(b, precision, b->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(NLSE_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->gprec = precision;
+    b->gchunk = chunk;
+  }
+  if ((rc = batch_row(b, row_flags(precision, 0), s))) return rc;
+  for (int64_t r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(b->gexec, s));
+  for (int64_t k = reps * chunk; k < nsteps; ++k) {
+    if ((rc = batch_col(b, s))) return rc;
+    if ((rc = batch_row(b, row_flags(precision, k == nsteps - 1 ? 2 : 1), s))) return rc;
+  }
+  return NLSE_OK;
+}
+
+int nlse_batch_run_pass(nlse_batch* b, void* base, int pass, int64_t count, void* stream) {
+  if (!b || !base || count < 0 || (pass != 0 && pass != 1)) return fail(NLSE_ERR_ARG, "bad batch pass arguments");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = batch_bind(b, base, s);
+  BatchFlags f = row_flags(0, 1);
+  f.timing = 1;
+  for (int64_t k = 0; k < count && !rc; ++k) rc = pass == 0 ? batch_row(b, f, s) : batch_col(b, s, 1);
+  return rc;
+}
+
+}  // extern "C"

@@ -49,6 +49,10 @@ static const LineOps* line_ops(int64_t n) {
   return &ops[l - 3];
 }
 
+namespace nlse {
+const LineOps* line_ops_public(int64_t n) { return line_ops(n); }
+}  // namespace nlse
+
 static RowArgs row_defaults() {
   RowArgs a;
   std::memset(&a, 0, sizeof a);

@@ -214,6 +214,9 @@ LineOps make_ops() {
 }
 
 
+// Dispatch table entry of a line length (engine.hip), or null.
+const LineOps* line_ops_public(int64_t n);
+
 // Defined in the lines_*.hip units.
 LineOps make_ops_8();
 LineOps make_ops_16();

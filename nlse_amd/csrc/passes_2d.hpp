@@ -48,16 +48,22 @@ struct RowArgs {
   float out_scale;        // applied to the stored result (1 or 1/N)
 };
 
+// LDS floats2 of one row block.
+template <int N, int LPB, int G>
+constexpr int row_lds_size() {
+  return LPB * G * LdsLine<N, 4, 0>::SIZE;
+}
+
+// Row sweep body: LPB line groups starting at group `g0`.
 template <int N, int P, int LPB, int G>
-__global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
+__device__ __forceinline__ void row_body(const RowArgs& a, int64_t g0, float2* lds) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
   constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
-  __shared__ float2 lds[LPB * G * LINE];
 
   const int li = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int64_t gi = (int64_t)blockIdx.x * LPB + li;
+  const int64_t gi = g0 + li;
   const bool active = gi < a.ngroups;
   const int64_t gsafe = active ? gi : 0;
   const int64_t item = gsafe / a.rows_per_item;
@@ -149,6 +155,12 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
   }
 }
 
+template <int N, int P, int LPB, int G>
+__global__ __launch_bounds__(LPB*(N / P)) void k_row(RowArgs a) {
+  __shared__ float2 lds[row_lds_size<N, LPB, G>()];
+  row_body<N, P, LPB, G>(a, (int64_t)blockIdx.x * LPB, lds);
+}
+
 struct ColArgs {
   float2* A;
   int64_t nslabs;      // items * components
@@ -175,25 +187,27 @@ struct ColArgs {
 // thread owning G columns (c0 + g*WT + cw) so that twiddles and the y
 // factor of the propagator are shared; LDS exchanges run in rounds of GR
 // columns per thread column.
-template <int N, int P, int WT, int G, int GR>
-__global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
+// LDS float2s of one column block: one float2 of padding every 8 entries
+// and 4 more per line, so the lines of the WT thread columns start 8
+// banks apart.
+template <int N, int WT, int GR>
+constexpr int col_lds_size() {
+  return WT * GR * LdsLine<N, 3, 3>::SIZE;
+}
+
+// Column sweep body for tile `b` (tile index over slabs x tiles per slab).
+// PK >= 0 compiles only that propagator kind (fewer live registers); -1
+// dispatches on a.pkind.
+template <int N, int P, int WT, int G, int GR, int PK = -1>
+__device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* lds) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  // one float2 of padding every 8 entries and 4 more per line: the lines
-  // of the WT thread columns then start 8 banks apart
   constexpr int LINE = LdsLine<N, 3, 3>::SIZE;
   constexpr int W = WT * G;
-  __shared__ float2 lds[WT * GR * LINE];
 
   const int cw = threadIdx.x % WT;
   const int t = threadIdx.x / WT;
   const int tiles = a.NX / W;
-  // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
-  // so block b runs on XCD b % 8; give each XCD a contiguous range of
-  // tiles so that the two 64-byte halves of a 128-byte line are fetched
-  // by neighbouring tiles on the same XCD (one L2 fill instead of two)
-  const int64_t nblk = (int64_t)gridDim.x;
-  const int64_t b = (nblk % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
   const int64_t slab = b / tiles;
   const int c0 = (int)(b % tiles) * W + cw;
   const int64_t base = slab * (int64_t)N * a.NX + c0;
@@ -206,9 +220,10 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
     for (int g = 0; g < G; ++g) v[g][m] = a.A[base + (int64_t)(t + m * T) * a.NX + g * WT];
 
   if (a.do_fwd) fft_line<N, P, G, false, GR, 3>(v, t, a.tw, lline, LINE);
+  const int pkind = PK >= 0 ? PK : a.pkind;
   if (a.do_prop) {
     const int comp = (int)(slab % a.ncomp);
-    if (a.pkind == PROP_SEP64) {
+    if (pkind == PROP_SEP64) {
       double2 px[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) px[g] = a.px[comp * a.NX + c0 + g * WT];
@@ -218,7 +233,7 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep_c64(v[g][m], px[g], py, a.scale);
       }
-    } else if (a.pkind == PROP_SEP3D) {
+    } else if (PK < 0 && pkind == PROP_SEP3D) {
       // column c = x*NT + t of an (NY, NX, NT) field: complex64(px*py)
       // times the complex128 temporal factor, applied in FP64
       double2 px[G], pt[G];
@@ -234,7 +249,7 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep3d(v[g][m], px[g], py, pt[g], (double)a.scale);
       }
-    } else if (a.pkind == PROP_KREAL) {
+    } else if (PK < 0 && pkind == PROP_KREAL) {
       // even real spectrum: entry (ky, kx) of the quadrant table at
       // (min(ky, N-ky), min(kx, NX-kx))
       const int hx = a.NX / 2 + 1;
@@ -254,14 +269,14 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
           v[g][m] = make_float2(v[g][m].x * f, v[g][m].y * f);
         }
       }
-    } else if (a.pkind == PROP_C128) {
+    } else if (PK < 0 && pkind == PROP_C128) {
       const double2* pt = a.ptab128 + c0;
 #pragma unroll
       for (int m = 0; m < P; ++m)
 #pragma unroll
         for (int g = 0; g < G; ++g)
           v[g][m] = prop_c128(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * WT], (double)a.scale);
-    } else {
+    } else if (PK < 0 || PK == PROP_C64) {
       const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c0;
 #pragma unroll
       for (int m = 0; m < P; ++m)
@@ -281,6 +296,18 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   for (int m = 0; m < P; ++m)
 #pragma unroll
     for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
+}
+
+template <int N, int P, int WT, int G, int GR>
+__global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
+  __shared__ float2 lds[col_lds_size<N, WT, GR>()];
+  // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+  // so block b runs on XCD b % 8; give each XCD a contiguous range of
+  // tiles so that the two 64-byte halves of a 128-byte line are fetched
+  // by neighbouring tiles on the same XCD (one L2 fill instead of two)
+  const int64_t nblk = (int64_t)gridDim.x;
+  const int64_t b = (nblk % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+  col_body<N, P, WT, G, GR>(a, b, lds);
 }
 
 // ----------------------------------------------------------------- 1-D
