@@ -229,6 +229,9 @@ typedef struct {
 } nlse_batch_item;
 int nlse_batch_create(const nlse_batch_item* items, int64_t count, int loss_saturates, nlse_batch** out);
 int nlse_batch_destroy(nlse_batch* batch);
+/* Persistent grid of the batch sweeps: blocks per CU of the row and column
+ * kernels (defaults 4 and 2; chosen by the autotuner). */
+int nlse_batch_set_grid(nlse_batch* batch, int row_blocks_per_cu, int col_blocks_per_cu);
 /* Work units and persistent grid sizes of the row and column sweeps. */
 int nlse_batch_config(const nlse_batch* batch, int64_t* row_units, int64_t* col_units, int* row_grid,
                       int* col_grid);

@@ -92,6 +92,7 @@ SIGNATURES = {
     "nlse_batch_create": (c_int, [c_void_p, c_i64, c_int, c_void_p]),
     "nlse_batch_destroy": (c_int, [c_void_p]),
     "nlse_batch_config": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nlse_batch_set_grid": (c_int, [c_void_p, c_int, c_int]),
     "nlse_batch_run": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
     "nlse_batch_run_pass": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_void_p]),
     "nlse_plan_config": (c_int, [c_void_p, C.POINTER(c_int), C.POINTER(c_int)]),

@@ -1,0 +1,161 @@
+"""Autotuner of the engine's launch choices, with a versioned on-disk cache.
+
+The reference tunes per call: pyFFTW plans with a wisdom file
+(nlse.py:232-254) and VkFFT's ``tune=True`` (nlse.py:216).  Here the
+choices that the engine leaves open are timed once per (GPU architecture,
+problem shape, flags) on the real tables and remembered in a JSON file:
+
+* plans (2-D and 3-D): the column tile (columns per block of the column
+  sweep, i.e. the LDS tile and the columns each thread carries) and the
+  number of steady steps per captured hipGraph (0: direct launches);
+* uneven batches: the persistent grid of the row and column sweeps
+  (blocks per CU) and the graph chunk.
+
+The FFT radix plan is fixed by the line length (radix 16 with one 2/4/8
+remainder stage); every candidate computes bit-identical results, so the
+tuner only changes time.
+
+Cache: $NLSE_AMD_TUNE_CACHE, else ~/.cache/nlse_amd/tune-v1.json.  With
+NLSE_AMD_AUTOTUNE=1 the engine tunes a shape on first use; otherwise a
+shape is tuned by NLSE.tune(E) / NLSEBatch.tune(E_list) and cached entries
+are used whenever present.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+
+VERSION = 1
+COL_WIDTHS = (1, 2, 4, 8, 16, 32)
+GRAPH_CHUNKS = (0, 8, 16, 32)
+BATCH_GRIDS = ((2, 1), (4, 2), (8, 2), (4, 1), (8, 4))
+
+
+def cache_path() -> str:
+    p = os.environ.get("NLSE_AMD_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "nlse_amd", f"tune-v{VERSION}.json")
+
+
+def enabled() -> bool:
+    """Tune automatically on first use of a shape (NLSE_AMD_AUTOTUNE=1)."""
+    return os.environ.get("NLSE_AMD_AUTOTUNE", "0") not in ("", "0")
+
+
+def load() -> dict:
+    try:
+        with open(cache_path()) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    if not isinstance(data, dict) or data.get("version") != VERSION:
+        return {}
+    return data.get("entries", {})
+
+
+def save(entries: dict) -> None:
+    path = cache_path()
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    tmp = path + f".tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump({"version": VERSION, "entries": entries}, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def lookup(key: str):
+    return load().get(key)
+
+
+def store(key: str, value: dict) -> None:
+    entries = load()
+    entries[key] = value
+    save(entries)
+
+
+def arch() -> str:
+    import torch
+
+    try:
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    except Exception:  # noqa: BLE001
+        return "unknown"
+
+
+def plan_key(geo: dict, loss_saturates: bool, nonlocal_shape) -> str:
+    """Cache key of a plan: architecture, geometry, flags."""
+    nl = "x".join(map(str, nonlocal_shape)) if nonlocal_shape is not None else "0"
+    return (f"plan|{arch()}|nd{geo['ndim']}|{geo['ny']}x{geo['nx']}x{geo['nt']}|items{geo['items']}"
+            f"|c{geo['ncomp']}|nl{nl}|ls{int(bool(loss_saturates))}")
+
+
+def batch_key(shapes) -> str:
+    from collections import Counter
+
+    hist = ",".join(f"{ny}x{nx}:{n}" for (ny, nx), n in sorted(Counter(shapes).items()))
+    return f"batch|{arch()}|{hist}"
+
+
+def _time_run(run, A0, nsteps: int, reps: int = 2) -> float:
+    """Best seconds per step of run(A, nsteps) over `reps` repetitions."""
+    import torch
+
+    A = A0.clone()
+    run(A, 2)
+    best = float("inf")
+    for _ in range(reps):
+        A.copy_(A0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(A, nsteps)
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) / nsteps)
+    return best
+
+
+def tune_plan(make_engine, A0, precision: int, nsteps: int = 40) -> dict:
+    """Time every column tile, then every graph chunk of the best tile.
+    make_engine(col_width) returns a configured engine (Plan)."""
+    results = {}
+    seen = set()
+    for w in COL_WIDTHS:
+        eng = make_engine(w)
+        eff = eng.config()[1]
+        if eff in seen:
+            continue
+        seen.add(eff)
+        eng.graph_chunk = 16
+        results[eff] = _time_run(lambda A, n: eng.run(A, n, precision), A0, nsteps)
+    best_w = min(results, key=results.get)
+    eng = make_engine(best_w)
+    chunks = {}
+    for c in GRAPH_CHUNKS:
+        eng.graph_chunk = c
+        chunks[c] = _time_run(lambda A, n: eng.run(A, n, precision), A0, nsteps)
+    best_c = min(chunks, key=chunks.get)
+    return {"col_width": int(best_w), "graph_chunk": int(best_c), "step_us": chunks[best_c] * 1e6,
+            "col_width_us": {str(k): v * 1e6 for k, v in results.items()},
+            "graph_chunk_us": {str(k): v * 1e6 for k, v in chunks.items()}}
+
+
+def tune_batch(eng, buf, precision: int, nsteps: int = 20) -> dict:
+    """Time the persistent grids and graph chunks of a batch engine."""
+    grids = {}
+    for rb, cb in BATCH_GRIDS:
+        eng.set_grid(rb, cb)
+        eng.graph_chunk = 16
+        grids[(rb, cb)] = _time_run(lambda A, n: eng.run(A, n, precision), buf, nsteps)
+    best_g = min(grids, key=grids.get)
+    eng.set_grid(*best_g)
+    chunks = {}
+    for c in GRAPH_CHUNKS:
+        eng.graph_chunk = c
+        chunks[c] = _time_run(lambda A, n: eng.run(A, n, precision), buf, nsteps)
+    best_c = min(chunks, key=chunks.get)
+    eng.graph_chunk = best_c
+    return {"row_blocks_per_cu": best_g[0], "col_blocks_per_cu": best_g[1], "graph_chunk": int(best_c),
+            "step_us": chunks[best_c] * 1e6,
+            "grid_us": {f"{k[0]},{k[1]}": v * 1e6 for k, v in grids.items()},
+            "graph_chunk_us": {str(k): v * 1e6 for k, v in chunks.items()}}

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _device as D
-from . import _lib
+from . import _lib, autotune
 from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE
 from .nlse import NLSE, count_steps
 
@@ -72,6 +72,10 @@ class BatchEngine:
                    "batch_create")
         self._h = h
         self.graph_chunk = 16
+
+    def set_grid(self, row_blocks_per_cu: int, col_blocks_per_cu: int) -> None:
+        _lib.check(_lib.load().nlse_batch_set_grid(self._h, int(row_blocks_per_cu), int(col_blocks_per_cu)),
+                   "batch_set_grid")
 
     def config(self) -> dict:
         a, b = C.c_int64(), C.c_int64()
@@ -154,12 +158,24 @@ class NLSEBatch:
             raise ValueError("all instances of a batch take the same number of steps")
         eng = BatchEngine(self._items(precision, z), self.sims[0].loss_saturates)
         eng.graph_chunk = self.graph_chunk
+        tuned = autotune.lookup(autotune.batch_key([(s.NY, s.NX) for s in self.sims]))
+        if tuned is not None:
+            eng.set_grid(tuned["row_blocks_per_cu"], tuned["col_blocks_per_cu"])
+            eng.graph_chunk = int(tuned["graph_chunk"])
         buf = torch.empty(eng.total, dtype=torch.complex64, device="cuda")
         for s, E, v in zip(self.sims, E_list, eng.views(buf)):
             assert tuple(E.shape) == (s.NY, s.NX), "Shape mismatch"
             A, _ = s._prepare_output_array(E, normalize)
             v.copy_(A)
         return eng, buf, nsteps.pop()
+
+    def tune(self, E_list, precision: str = "single") -> dict:
+        """Time the persistent grids and graph chunks of this batch's
+        shapes and cache the fastest (nlse_amd.autotune)."""
+        eng, buf, _ = self.prepare(E_list, 1e-30, precision)
+        res = autotune.tune_batch(eng, buf, self.sims[0]._precision_flag(precision))
+        autotune.store(autotune.batch_key([(s.NY, s.NX) for s in self.sims]), res)
+        return res
 
     def out_field(self, E_list, z: float, precision: str = "single", normalize: bool = True) -> list:
         """Propagate every field over z; returns the fields in the input's

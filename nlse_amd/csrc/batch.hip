@@ -329,6 +329,19 @@ int nlse_batch_destroy(nlse_batch* b) {
   return NLSE_OK;
 }
 
+int nlse_batch_set_grid(nlse_batch* b, int row_blocks_per_cu, int col_blocks_per_cu) {
+  if (!b || row_blocks_per_cu < 1 || col_blocks_per_cu < 1 || row_blocks_per_cu > 64 || col_blocks_per_cu > 64)
+    return fail(NLSE_ERR_ARG, "bad batch grid (%d, %d blocks per CU)", row_blocks_per_cu, col_blocks_per_cu);
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  for (int li = 0; li < NLEN; ++li) {
+    b->row_grid[li] = (int)std::min<int64_t>(b->row_off[li + 1] - b->row_off[li], (int64_t)ncu * row_blocks_per_cu);
+    b->col_grid[li] = (int)std::min<int64_t>(b->col_off[li + 1] - b->col_off[li], (int64_t)ncu * col_blocks_per_cu);
+  }
+  batch_graph_free(b);  // the captured launches bake in the grid
+  return NLSE_OK;
+}
+
 int nlse_batch_config(const nlse_batch* b, int64_t* row_units, int64_t* col_units, int* row_grid, int* col_grid) {
   if (!b) return fail(NLSE_ERR_ARG, "null batch");
   if (row_units) *row_units = b->n_row;

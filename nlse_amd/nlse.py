@@ -19,6 +19,7 @@ from scipy import special
 from scipy.constants import c, epsilon_0
 
 from . import _device as D
+from . import autotune
 from . import _lib
 from . import kernels_hip
 from .engine import NL1_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, Plan, inv_or_zero
@@ -282,14 +283,57 @@ class NLSE:
         items = int(np.prod(self._param_lead(A))) if self._param_lead(A) else 1
         return dict(ndim=2, ny=self.NY, nx=self.NX, nt=1, items=items, ncomp=self._ncomp())
 
-    def _engine_for(self, A: torch.Tensor) -> Plan:
+    def _tune_key(self, geo: dict) -> str:
+        shape = tuple(np.shape(D.to_host(self.nl_profile))) if self.nl_length > 0 else None
+        return autotune.plan_key(geo, self.loss_saturates, shape)
+
+    def _engine_for(self, A: torch.Tensor, col_width: int | None = None) -> Plan:
+        """The fused plan for A's geometry.  The column tile is the
+        attribute col_width when set, else the autotuner's cached choice
+        for this geometry, else the engine default."""
         geo = self._engine_geometry(A)
-        key = tuple(sorted(geo.items())) + (bool(self.loss_saturates), int(self.col_width))
+        chunk = None
+        if col_width is None:
+            col_width = int(self.col_width)
+            if col_width == 0 and geo["ndim"] != 1:
+                tuned = autotune.lookup(self._tune_key(geo))
+                if tuned is not None:
+                    col_width, chunk = int(tuned["col_width"]), int(tuned["graph_chunk"])
+        key = tuple(sorted(geo.items())) + (bool(self.loss_saturates), int(col_width))
         if self._engine is None or self._engine_key != key:
             self._engine = Plan(geo["ny"], geo["nx"], geo["items"], geo["ncomp"], self.loss_saturates,
-                                self.col_width, ndim=geo["ndim"], nt=geo["nt"])
+                                col_width, ndim=geo["ndim"], nt=geo["nt"])
             self._engine_key = key
+        if chunk is not None:
+            self._engine.graph_chunk = chunk
         return self._engine
+
+    def tune(self, E_in, precision: str = "single") -> dict:
+        """Time the engine's launch choices (column tile, graph chunk) for
+        this instance's geometry on its own tables and cache the fastest
+        (nlse_amd.autotune); later engines of this geometry use them."""
+        if self.propagator is None:
+            self._own_propagator()
+        self._send_arrays_to_gpu()
+        try:
+            V = None if self.V is None else self.V.clone()
+            A, _ = self._prepare_output_array(E_in, True)
+            geo = self._engine_geometry(A)
+            if geo["ndim"] == 1:
+                return {}
+            flag = self._precision_flag(precision)
+
+            def make(w):
+                eng = self._engine_for(A, col_width=w)
+                self._configure(eng, A, V, self.propagator, precision)
+                return eng
+
+            res = autotune.tune_plan(make, A, flag)
+            autotune.store(self._tune_key(geo), res)
+            self._engine = None
+            return res
+        finally:
+            self._retrieve_arrays_from_gpu()
 
     def _nl_records(self, dz: float, items_lead: tuple) -> np.ndarray:
         """Per-realisation parameters of nl_prop[_without_V] for one
@@ -361,6 +405,15 @@ class NLSE:
         eng.run(A, 1, self._precision_flag(precision), 0)
 
     def _run_chunk(self, A, V, nsteps: int, precision: str, step0: int) -> None:
+        if autotune.enabled() and nsteps >= 64 and self.col_width == 0:
+            geo = self._engine_geometry(A)
+            if geo["ndim"] != 1 and autotune.lookup(self._tune_key(geo)) is None:
+                def make(w):
+                    eng = self._engine_for(A, col_width=w)
+                    self._configure(eng, A, V, self.propagator, precision)
+                    return eng
+
+                autotune.store(self._tune_key(geo), autotune.tune_plan(make, A, self._precision_flag(precision)))
         eng = self._engine_for(A)
         self._configure(eng, A, V, self.propagator, precision)
         eng.run(A, nsteps, self._precision_flag(precision), step0)
