@@ -116,8 +116,9 @@ def kernel_bytes(P: int, N: int, r: int, local: bool) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2", "c3"), default="c1",
-                    help="c1: 2048^2 nonlocal split step (the metric); c2: uneven batch; c3: 3-D with GVD")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4"), default="c1",
+                    help="c1: 2048^2 nonlocal split step (the metric); c2: uneven batch; c3: 3-D with GVD; "
+                         "c4: learned corrector")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--grid", type=int, default=2048)
@@ -130,13 +131,15 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every GPU box here has one "
               "GPU; the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
         return 2
-    defaults = {"c1": (1000, 50), "c2": (500, 20), "c3": (200, 10)}[args.workload]
+    defaults = {"c1": (1000, 50), "c2": (500, 20), "c3": (200, 10), "c4": (100, 5)}[args.workload]
     args.steps = defaults[0] if args.steps is None else args.steps
     args.warmup = defaults[1] if args.warmup is None else args.warmup
     if args.workload == "c2":
         return bench_c2(args)
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload == "c4":
+        return bench_c4(args)
     return bench_c1(args)
 
 
@@ -398,6 +401,109 @@ def bench_c3(args) -> int:
         "step_roofline": {"B_per_point": 64, "min_B_per_point": 48, "frac_of_min_schedule": sps * P * 48 / HBM_PEAK},
         "grid_point_steps_per_s": sps * P,
         "energy_drift_per_step": (e_end - e_start) / e_start / (args.warmup + args.steps),
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_c4(args) -> int:
+    """Learned corrector: accuracy at 128^2 (trained there) against the FP64
+    oracle over L = 10 mm, and throughput at 1024^2 of the corrected coarse
+    loop (coarse step dz_c = 8 dz + network) against the fine-step loop."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from nlse_amd import NLSE
+    from nlse_amd import corrector as CR
+    from nlse_amd.fno import FNO
+
+    t0 = time.perf_counter()
+    pairs = CR.make_pairs(N=128, trajectories=80, coarse_steps=20, seed=1)
+    net = FNO(128, 128, width=32, layers=4, seed=0)
+    CR.train(net, pairs, steps=800, batch=8, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    held = CR.make_pairs(N=128, trajectories=4, coarse_steps=20, seed=99)
+    per_step = CR.evaluate(net, held)
+    del pairs
+    # full-length runs against the FP64 oracle (complex128 scheme)
+    acc = []
+    z = 10e-3
+    for sd in (123, 124):
+        rng = np.random.default_rng(sd)
+        mat = CR.material(rng)
+        kw = dict(alpha=mat["alpha"], power=mat["power"], window=CR.WINDOW, n2=mat["n2"], V=None, L=10e-3,
+                  NX=128, NY=128, Isat=mat["Isat"])
+        s = NLSE(**kw)
+        s.delta_z = 1e-5
+        E0 = CR._speckle_field(s, rng)
+        o = oracle.NLSE(mode="c128", **kw)
+        o.delta_z = 1e-5
+        ref = o.out_field(E0.astype(np.complex128), z)
+        fine = s.out_field(E0, z, verbose=False)
+        sc = NLSE(**kw)
+        sc.delta_z = 8e-5
+        coarse = sc.out_field(E0, z, verbose=False)
+        s2 = NLSE(**kw)
+        s2.delta_z = 1e-5
+        corr, st = CR.CorrectedSolver(s2, net, drift_tol=1e-3).out_field(E0, z)
+        e = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))  # noqa: E731
+        acc.append({"seed": sd, "fine": e(fine), "coarse": e(coarse), "corrected": e(corr),
+                    "fallbacks": st["fallbacks"], "coarse_steps": st["coarse_steps"]})
+    # throughput at 1024^2 (timing only: the network's weights do not change it)
+    N = 1024
+    rng = np.random.default_rng(5)
+    mat = CR.material(rng, scale=False)
+    sf, sc = CR._engines(N, mat, 1e-5, 8)
+    E0 = CR._speckle_field(sf, rng)
+    A, ef = CR._prepared(sf, E0)
+    _, ec = CR._prepared(sc, E0)
+    big = FNO(N, N, width=32, layers=4, seed=0)
+    big.res_scale = net.res_scale
+    ef.run(A, 16)
+    t_fine = _timed(ef.run, A, 8 * args.steps) / (8 * args.steps)
+    Y = torch.empty_like(A)
+    B = torch.empty_like(A)
+
+    def corrected(n):
+        for _ in range(n):
+            Y.copy_(A)
+            ec.run(Y, 1)
+            big.correct(Y[None], A[None], out=B[None])
+            A.copy_(B)
+
+    corrected(2)
+    t_corr = _timed(corrected, args.steps) / args.steps
+    t_coarse = _timed(ec.run, A, args.steps) / args.steps
+    x_in = torch.zeros(1, 4, N, N, device="cuda")
+    big.forward(x_in)
+    t_net = _timed(big.forward, x_in)
+    fine_eq = 8.0 / t_corr
+    result = {
+        "metric": "fine-step equivalents/s, corrected coarse loop vs plain fine steps (1024^2 c64)",
+        "value": fine_eq,
+        "unit": "fine-step-equivalents/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_corr * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 solver, bf16 MFMA network (fp32 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": "config-4: FNO corrector (4 spectral layers, 32 channels, 32x32 modes) of coarse "
+                               "steps dz_c = 8e-5 vs fine dz = 1e-5; trained at 128^2 on 1600 pairs from the "
+                               "plain solver (config-0 material x U(0.5,1.5), speckle inputs); throughput at "
+                               "1024^2",
+                   "train_seconds": t_train, "train_pairs": 1600},
+        "fine_steps_per_s": 1.0 / t_fine,
+        "coarse_steps_per_s": 1.0 / t_coarse,
+        "network_ms": t_net * 1e3,
+        "speedup_vs_fine": fine_eq * t_fine,
+        "per_coarse_step_error": per_step,
+        "full_length_vs_fp64_oracle_128": acc,
     }
     print(json.dumps(result), flush=True)
     return 0

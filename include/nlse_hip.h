@@ -243,6 +243,42 @@ int nlse_batch_run(nlse_batch* batch, void* base_dev, int64_t nsteps, int precis
 /* Timing only: `count` row (pass 0) or column (pass 1) sweeps of the batch. */
 int nlse_batch_run_pass(nlse_batch* batch, void* base_dev, int pass, int64_t count, void* stream);
 
+/* ------------------------------------------------------------------
+ * Learned corrector kernels (no counterpart in the reference).  A Fourier
+ * neural operator maps a coarse split step to the fine-step result; its
+ * products run on the BF16 matrix cores with FP32 accumulation.  All
+ * pointers are device pointers, all calls asynchronous on `stream`. */
+/* C(m,n) = alpha sum_k A(m,k) B(k,n) + beta C(m,n) [+ bias[n]], batched:
+ * A(m,k) = A[b*sa + m*sam + k*sak], B(k,n) = B[b*sb + k*sbk + n*sbn],
+ * C(m,n) = C[b*sc + m*scm + n*scn].  Operands FP32 in memory, rounded to
+ * BF16 on load.  act != NULL also stores gelu(C) at the same offsets. */
+int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, int64_t sak, int64_t sa,
+                  const float* B, int64_t sbk, int64_t sbn, int64_t sb, float* C, int64_t scm, int64_t scn,
+                  int64_t sc, float alpha, float beta, const float* bias, float* act, void* stream);
+/* Per-mode complex channel mixing of a spectral layer (32 x 32 modes):
+ * P1 [B*Cin][64][64] -> Q [B*Cout][64][64] with R [1024][Cout][Cin][2]. */
+int nlse_fno_mix(const float* P1, float* Q, const float* R, int B, int Cin, int Cout, void* stream);
+/* Adjoint of nlse_fno_mix: dP1 and dR (summed over the batch) from dQ. */
+int nlse_fno_mix_bwd(const float* P1, const float* dQ, const float* R, float* dP1, float* dR, int B, int Cin,
+                     int Cout, void* stream);
+int nlse_fno_gelu_bwd(const float* z, const float* dy, float* dz, int64_t n, void* stream);
+/* out[c] = sum_r x[r*sr + c*sc], r < rows, c < cols (FP64 partials). */
+int nlse_fno_colsum(const float* x, int64_t rows, int64_t cols, int64_t sr, int64_t sc, float* out, void* stream);
+/* out[i] = sum_p x[p*n + i] (split-K partial products). */
+int nlse_fno_sum_parts(const float* x, int parts, int64_t n, float* out, void* stream);
+/* Network input [B][4][hw] = [Re Y, Im Y, Re X, Im X] * inv_s[b] (Y, X complex64). */
+int nlse_fno_pack(const void* Y, const void* X, const float* inv_s, float* out, int B, int64_t hw, void* stream);
+/* Training target [B][2][hw] = (F - Y) * inv_s[b]. */
+int nlse_fno_residual(const void* F, const void* Y, const float* inv_s, float* tgt, int B, int64_t hw,
+                      void* stream);
+/* A = Y + s[b] (out_re + i out_im), out [B][2][hw]. */
+int nlse_fno_apply(const void* Y, const float* out, const float* s, void* A, int B, int64_t hw, void* stream);
+/* d = 2 (out - tgt) / n and FP64 partial sums of (out - tgt)^2 in part[nparts]. */
+int nlse_fno_mse(const float* out, const float* tgt, float* d, int64_t n, double* part, int nparts, void* stream);
+/* One Adam step (bias-corrected, step t >= 1) on n parameters. */
+int nlse_fno_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+                  int64_t t, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

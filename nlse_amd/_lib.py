@@ -93,6 +93,20 @@ SIGNATURES = {
     "nlse_batch_destroy": (c_int, [c_void_p]),
     "nlse_batch_config": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nlse_batch_set_grid": (c_int, [c_void_p, c_int, c_int]),
+    "nlse_fno_gemm": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_i64, c_i64, c_i64, c_void_p, c_i64, c_i64,
+                              c_i64, c_void_p, c_i64, c_i64, c_i64, C.c_float, C.c_float, c_void_p, c_void_p,
+                              c_void_p]),
+    "nlse_fno_mix": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "nlse_fno_mix_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "nlse_fno_gelu_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_void_p]),
+    "nlse_fno_colsum": (c_int, [c_void_p, c_i64, c_i64, c_i64, c_i64, c_void_p, c_void_p]),
+    "nlse_fno_sum_parts": (c_int, [c_void_p, c_int, c_i64, c_void_p, c_void_p]),
+    "nlse_fno_pack": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_i64, c_void_p]),
+    "nlse_fno_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_i64, c_void_p]),
+    "nlse_fno_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_i64, c_void_p]),
+    "nlse_fno_mse": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_void_p, c_int, c_void_p]),
+    "nlse_fno_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, C.c_float, C.c_float, C.c_float,
+                              C.c_float, c_i64, c_void_p]),
     "nlse_batch_run": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
     "nlse_batch_run_pass": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_void_p]),
     "nlse_plan_config": (c_int, [c_void_p, C.POINTER(c_int), C.POINTER(c_int)]),

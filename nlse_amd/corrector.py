@@ -1,0 +1,180 @@
+"""Training data, training loop and the corrected propagation loop of the
+learned corrector (nlse_amd.fno).
+
+  pairs = make_pairs(N=128, trajectories=16, coarse_steps=24)   # plain solver
+  net = FNO(128, 128)
+  train(net, pairs, steps=2000)
+  out, stats = CorrectedSolver(sim, net).out_field(E0, z)
+
+Pairs: (X, Y, F) with X a field along a trajectory of the plain fine-step
+solver, Y one coarse split step (dz_c = R dz) of X, F the R fine steps of X.
+Material parameters are the config-0 values scaled by U(0.5, 1.5) per
+trajectory (n2, power, Isat, alpha); inputs are Gaussian beams with a
+random-phase speckle (seeded).
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .fno import FNO
+from .nlse import NLSE, count_steps
+
+WAIST = 2.23e-3
+WINDOW = 4 * WAIST
+
+
+def _speckle_field(sim, rng) -> np.ndarray:
+    ny, nx = sim.XX.shape
+    phi = rng.uniform(0.0, 2 * np.pi, (ny, nx))
+    band = rng.uniform(1 / 16, 1 / 4)
+    mask = (np.abs(np.fft.fftfreq(nx))[None, :] <= band / 2) & (np.abs(np.fft.fftfreq(ny))[:, None] <= band / 2)
+    S = np.fft.ifft2(np.fft.fft2(np.exp(1j * phi)) * mask)
+    S /= np.sqrt(np.mean(np.abs(S) ** 2))
+    G = np.exp(-(sim.XX.astype(np.float64) ** 2 + sim.YY.astype(np.float64) ** 2) / (WAIST * rng.uniform(0.5, 1.0)) ** 2)
+    return (G * (1 + rng.uniform(0.0, 1.0) * S)).astype(np.complex64)
+
+
+def material(rng, scale: bool = True) -> dict:
+    """Config-0 material parameters, each scaled by U(0.5, 1.5)."""
+    f = (lambda: rng.uniform(0.5, 1.5)) if scale else (lambda: 1.0)
+    return dict(alpha=20 * f(), power=1.05 * f(), n2=-1.6e-9 * f(), Isat=1e5 * f())
+
+
+def _engines(N: int, mat: dict, dz: float, ratio: int):
+    sims = []
+    for step in (dz, ratio * dz):
+        s = NLSE(mat["alpha"], mat["power"], WINDOW, mat["n2"], None, 10e-3, NX=N, NY=N, Isat=mat["Isat"])
+        s.delta_z = step
+        sims.append(s)
+    return sims
+
+
+def _prepared(sim, E0):
+    sim._own_propagator()
+    sim._send_arrays_to_gpu()
+    A, _ = sim._prepare_output_array(E0, True)
+    eng = sim._engine_for(A)
+    sim._configure(eng, A, sim.V, sim.propagator, "single")
+    return A, eng
+
+
+def make_pairs(N: int = 128, trajectories: int = 16, coarse_steps: int = 24, ratio: int = 8, dz: float = 1e-5,
+               seed: int = 0, scale: bool = True) -> dict:
+    """Stacks X, Y, F (complex64, [count][N][N]) on the device."""
+    rng = np.random.default_rng(seed)
+    X, Y, F = [], [], []
+    for t in range(trajectories):
+        mat = material(rng, scale)
+        sf, sc = _engines(N, mat, dz, ratio)
+        E0 = _speckle_field(sf, rng)
+        A, ef = _prepared(sf, E0)
+        _, ec = _prepared(sc, E0)
+        for k in range(coarse_steps):
+            X.append(A.clone())
+            y = A.clone()
+            ec.run(y, 1)
+            Y.append(y)
+            ef.run(A, ratio)
+            F.append(A.clone())
+    return {"X": torch.stack(X), "Y": torch.stack(Y), "F": torch.stack(F), "ratio": ratio, "dz": dz, "N": N}
+
+
+def residual_scale(pairs: dict) -> float:
+    """RMS of the relative residual |F - Y| / |Y| over the pairs."""
+    return float(torch.sqrt(torch.mean(rel_err(pairs["Y"], pairs["F"]) ** 2 *
+                                       (torch.view_as_real(pairs["F"]).double().pow(2).flatten(1).sum(1) /
+                                        torch.view_as_real(pairs["Y"]).double().pow(2).flatten(1).sum(1)))))
+
+
+def train(net: FNO, pairs: dict, steps: int = 1000, batch: int = 8, lr: float = 1e-3, seed: int = 0,
+          log_every: int = 0) -> list:
+    """Adam on random mini-batches; cosine learning-rate decay.  Sets the
+    network's residual scale from the pairs.  Returns the loss history."""
+    rng = np.random.default_rng(seed)
+    net.res_scale = residual_scale(pairs)
+    n = pairs["X"].shape[0]
+    hist = []
+    for it in range(steps):
+        idx = torch.from_numpy(rng.choice(n, size=min(batch, n), replace=False)).to(pairs["X"].device)
+        Y = pairs["Y"].index_select(0, idx).contiguous()
+        X = pairs["X"].index_select(0, idx).contiguous()
+        F = pairs["F"].index_select(0, idx).contiguous()
+        lr_t = lr * 0.5 * (1 + np.cos(np.pi * it / max(1, steps)))
+        loss = net.train_step(Y, X, F, lr_t)
+        hist.append(loss)
+        if log_every and it % log_every == 0:
+            print(f"[train] step {it} loss {loss:.3e}", flush=True)
+    return hist
+
+
+def rel_err(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """Per-sample relative L2 error |a - b| / |b| (FP64)."""
+    d = torch.view_as_real(a - b).double().pow(2).flatten(1).sum(1)
+    r = torch.view_as_real(b).double().pow(2).flatten(1).sum(1)
+    return torch.sqrt(d / r)
+
+
+def evaluate(net: FNO, pairs: dict, batch: int = 8) -> dict:
+    """Mean relative error of the coarse step and of the corrected coarse
+    step against the R fine steps."""
+    e_c, e_n = [], []
+    n = pairs["X"].shape[0]
+    for i in range(0, n, batch):
+        Y = pairs["Y"][i:i + batch].contiguous()
+        X = pairs["X"][i:i + batch].contiguous()
+        F = pairs["F"][i:i + batch]
+        A = net.correct(Y, X)
+        e_c.append(rel_err(Y, F))
+        e_n.append(rel_err(A, F))
+    e_c = torch.cat(e_c)
+    e_n = torch.cat(e_n)
+    return {"coarse": float(e_c.mean()), "corrected": float(e_n.mean()),
+            "gain": float(e_c.mean() / e_n.mean())}
+
+
+class CorrectedSolver:
+    """Coarse steps of ratio * delta_z corrected by the network, with the
+    plain fine steps as fallback when the corrected step changes the power
+    by more than `drift_tol` (relative) from the coarse step's power."""
+
+    def __init__(self, sim: NLSE, net: FNO, ratio: int = 8, drift_tol: float = 1e-4):
+        self.sim, self.net, self.ratio, self.drift_tol = sim, net, ratio, drift_tol
+
+    def out_field(self, E0: np.ndarray, z: float):
+        import copy
+
+        sim = self.sim
+        fine = count_steps(z, sim.delta_z)
+        coarse = copy.copy(sim)
+        coarse._engine = None
+        coarse._propagator = None
+        coarse._prop_factors = None
+        coarse.delta_z = sim.delta_z * self.ratio
+        A, ef = _prepared(sim, E0)
+        _, ec = _prepared(coarse, E0)
+        stats = {"coarse_steps": fine // self.ratio, "fallbacks": 0, "fine_steps": fine % self.ratio}
+        Y = torch.empty_like(A)
+        B = torch.empty_like(A)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(stats["coarse_steps"]):
+            Y.copy_(A)
+            ec.run(Y, 1)
+            self.net.correct(Y[None], A[None], out=B[None])
+            p_y = torch.view_as_real(Y).double().pow(2).sum()
+            p_b = torch.view_as_real(B).double().pow(2).sum()
+            if float(torch.abs(p_b - p_y) / p_y) > self.drift_tol:
+                ef.run(A, self.ratio)
+                stats["fallbacks"] += 1
+            else:
+                A.copy_(B)
+        if stats["fine_steps"]:
+            ef.run(A, stats["fine_steps"])
+        torch.cuda.synchronize()
+        stats["seconds"] = time.perf_counter() - t0
+        sim._retrieve_arrays_from_gpu()
+        return A.cpu().numpy(), stats

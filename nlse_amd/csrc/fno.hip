@@ -1,0 +1,468 @@
+// Kernels of the learned split-step corrector (a Fourier neural operator):
+// a strided, batched GEMM on the BF16 matrix cores with FP32 accumulation,
+// the per-mode complex channel mixing of the spectral layers, and the
+// elementwise pieces of training (GELU, loss, bias gradients, Adam).
+//
+// Every large product of the network -- the truncated DFTs along x and y,
+// their inverses, the 1x1 channel mixes and all their adjoints -- is one
+// call of k_gemm_bf16: operands are FP32 in memory (activations and master
+// weights), rounded to BF16 when staged into LDS, multiplied with
+// v_mfma_f32_16x16x16_bf16 and accumulated in FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/nlse_hip.h"
+#include "common.hpp"
+
+namespace nlse {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short to_bf16(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+struct GemmArgs {
+  int M, N, K, batch;
+  const float* A;  // A(m, k) = A[b*sa + m*sam + k*sak]
+  int64_t sam, sak, sa;
+  const float* B;  // B(k, n) = B[b*sb + k*sbk + n*sbn]
+  int64_t sbk, sbn, sb;
+  float* C;        // C(m, n) = C[b*sc + m*scm + n*scn]
+  int64_t scm, scn, sc;
+  float alpha, beta;
+  const float* bias;  // [N] or null
+  float* act;         // if set: act(m, n) = gelu(C(m, n)) (same strides)
+};
+
+constexpr int GBM = 64, GBN = 64, GBK = 32;
+constexpr int GPAD = 4;  // bf16 padding per LDS row
+
+__device__ __forceinline__ float gelu_f(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float u = k0 * (x + k1 * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k0 * (1.f + 3.f * k1 * x * x);
+}
+
+// C = alpha * A B + beta * C [+ bias], 64x64 tile per block of 4 waves,
+// each wave a 32x32 quarter as 2x2 MFMA 16x16 tiles; K in steps of 32.
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+  __shared__ unsigned short As[GBM][GBK + GPAD];
+  __shared__ unsigned short Bs[GBN][GBK + GPAD];  // stored transposed: Bs[n][k]
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const int64_t bz = blockIdx.z;
+  const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  const float* A = g.A + bz * g.sa;
+  const float* B = g.B + bz * g.sb;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const bool a_kfast = g.sak == 1;
+  const bool b_nfast = g.sbn == 1;
+  for (int k0 = 0; k0 < g.K; k0 += GBK) {
+    // stage the A (64 x 32) and B (32 x 64) tiles, 8 elements per thread,
+    // lanes along the contiguous index of each operand
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + tid;
+      int mm, kk;
+      if (a_kfast) {
+        mm = e / GBK;
+        kk = e % GBK;
+      } else {
+        kk = e / GBM;
+        mm = e % GBM;
+      }
+      const int m = m0 + mm, k = k0 + kk;
+      const float v = (m < g.M && k < g.K) ? A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.f;
+      As[mm][kk] = to_bf16(v);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = r * 256 + tid;
+      int nn, kk;
+      if (b_nfast) {
+        kk = e / GBN;
+        nn = e % GBN;
+      } else {
+        nn = e / GBK;
+        kk = e % GBK;
+      }
+      const int n = n0 + nn, k = k0 + kk;
+      const float v = (n < g.N && k < g.K) ? B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.f;
+      Bs[nn][kk] = to_bf16(v);
+    }
+    __syncthreads();
+    // v_mfma_f32_16x16x16_bf16 operands: lane l holds A[l%16][4*(l/16)+i]
+    // and B[4*(l/16)+i][l%16]; the result lane l holds D[4*(l/16)+i][l%16]
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 16) {
+      bf16x4 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const unsigned short* pa = &As[wm + i * 16 + (lane & 15)][kk + 4 * (lane >> 4)];
+        const unsigned short* pb = &Bs[wn + i * 16 + (lane & 15)][kk + 4 * (lane >> 4)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[i][q] = (short)pa[q];
+          b[i][q] = (short)pb[q];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  float* C = g.C + bz * g.sc;
+  float* Act = g.act ? g.act + bz * g.sc : nullptr;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q;
+        const int n = n0 + wn + j * 16 + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        const int64_t off = (int64_t)m * g.scm + (int64_t)n * g.scn;
+        float v = g.alpha * acc[i][j][q];
+        if (g.beta != 0.f) v += g.beta * C[off];
+        if (g.bias) v += g.bias[n];
+        C[off] = v;
+        if (Act) Act[off] = gelu_f(v);
+      }
+}
+
+// ------------------------------------------------------- spectral mixing
+// Per sample b and retained mode (ky, kx) of a 32 x 32 set: the y-DFT
+// output P1 (64 x 64 per channel: rows cos/sin(ky), columns re/im(kx)) is
+// combined into the complex mode value, mixed over channels with the
+// complex weights R[mode][o][i], and written as the 64 x 64 block Q of the
+// inverse y-DFT input: Q = [[re, im], [-im, re]].
+struct MixArgs {
+  const float* P1;  // [B*Cin][64][64]
+  float* Q;         // [B*Cout][64][64]
+  const float* R;   // [1024][Cout][Cin][2] (re, im)
+  int B, Cin, Cout;
+};
+
+__device__ __forceinline__ void mode_value(const float* P, int ky, int kx, float& re, float& im) {
+  re = P[ky * 64 + kx] + P[(32 + ky) * 64 + 32 + kx];
+  im = P[ky * 64 + 32 + kx] - P[(32 + ky) * 64 + kx];
+}
+
+__global__ void k_fno_mix(MixArgs a) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)a.B * 1024 * a.Cout;
+  if (t >= total) return;
+  const int o = (int)(t % a.Cout);
+  const int mode = (int)((t / a.Cout) % 1024);
+  const int b = (int)(t / ((int64_t)a.Cout * 1024));
+  const int ky = mode >> 5, kx = mode & 31;
+  float sr = 0.f, si = 0.f;
+  for (int i = 0; i < a.Cin; ++i) {
+    float hr, hi;
+    mode_value(a.P1 + ((int64_t)b * a.Cin + i) * 4096, ky, kx, hr, hi);
+    const float* w = a.R + (((int64_t)mode * a.Cout + o) * a.Cin + i) * 2;
+    sr += w[0] * hr - w[1] * hi;
+    si += w[0] * hi + w[1] * hr;
+  }
+  float* Q = a.Q + ((int64_t)b * a.Cout + o) * 4096;
+  Q[ky * 64 + kx] = sr;
+  Q[ky * 64 + 32 + kx] = si;
+  Q[(32 + ky) * 64 + kx] = -si;
+  Q[(32 + ky) * 64 + 32 + kx] = sr;
+}
+
+// Adjoint of k_fno_mix: from dQ, the mode gradients dO = (dQ[re] + dQ[re'],
+// dQ[im] - dQ[-im]); dP1 (the adjoint of mode_value) from dH = R^H dO.
+struct MixBwdArgs {
+  const float* P1;  // forward input [B*Cin][64][64]
+  const float* dQ;  // [B*Cout][64][64]
+  const float* R;   // [1024][Cout][Cin][2]
+  float* dP1;       // [B*Cin][64][64]
+  float* dR;        // [1024][Cout][Cin][2]
+  int B, Cin, Cout;
+};
+
+__device__ __forceinline__ void mode_grad(const float* dQ, int ky, int kx, float& re, float& im) {
+  re = dQ[ky * 64 + kx] + dQ[(32 + ky) * 64 + 32 + kx];
+  im = dQ[ky * 64 + 32 + kx] - dQ[(32 + ky) * 64 + kx];
+}
+
+__global__ void k_fno_mix_bwd_input(MixBwdArgs a) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)a.B * 1024 * a.Cin;
+  if (t >= total) return;
+  const int i = (int)(t % a.Cin);
+  const int mode = (int)((t / a.Cin) % 1024);
+  const int b = (int)(t / ((int64_t)a.Cin * 1024));
+  const int ky = mode >> 5, kx = mode & 31;
+  float hr = 0.f, hi = 0.f;  // dH = sum_o conj(R[o][i]) dO[o]
+  for (int o = 0; o < a.Cout; ++o) {
+    float gr, gi;
+    mode_grad(a.dQ + ((int64_t)b * a.Cout + o) * 4096, ky, kx, gr, gi);
+    const float* w = a.R + (((int64_t)mode * a.Cout + o) * a.Cin + i) * 2;
+    hr += w[0] * gr + w[1] * gi;
+    hi += w[0] * gi - w[1] * gr;
+  }
+  float* d = a.dP1 + ((int64_t)b * a.Cin + i) * 4096;
+  d[ky * 64 + kx] = hr;
+  d[(32 + ky) * 64 + 32 + kx] = hr;
+  d[ky * 64 + 32 + kx] = hi;
+  d[(32 + ky) * 64 + kx] = -hi;
+}
+
+__global__ void k_fno_mix_bwd_weight(MixBwdArgs a) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)1024 * a.Cout * a.Cin;
+  if (t >= total) return;
+  const int i = (int)(t % a.Cin);
+  const int o = (int)((t / a.Cin) % a.Cout);
+  const int mode = (int)(t / ((int64_t)a.Cin * a.Cout));
+  const int ky = mode >> 5, kx = mode & 31;
+  float rr = 0.f, ri = 0.f;  // dR = sum_b dO[o] conj(H[i])
+  for (int b = 0; b < a.B; ++b) {
+    float gr, gi, hr, hi;
+    mode_grad(a.dQ + ((int64_t)b * a.Cout + o) * 4096, ky, kx, gr, gi);
+    mode_value(a.P1 + ((int64_t)b * a.Cin + i) * 4096, ky, kx, hr, hi);
+    rr += gr * hr + gi * hi;
+    ri += gi * hr - gr * hi;
+  }
+  float* d = a.dR + t * 2;
+  d[0] = rr;
+  d[1] = ri;
+}
+
+// ---------------------------------------------------------- elementwise
+__global__ void k_fno_gelu_bwd(const float* __restrict__ z, const float* __restrict__ dy, float* __restrict__ dz,
+                               int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dz[i] = dy[i] * gelu_grad(z[i]);
+}
+
+// Column sums: out[c] = sum over rows r of x[r*sr + c*sc] for `rows` rows
+// (bias gradients; one block per column, FP64 partials).
+__global__ void k_fno_colsum(const float* __restrict__ x, int64_t rows, int64_t sr, int64_t sc, float* out) {
+  __shared__ double part[256];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += x[r * sr + (int64_t)c * sc];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = (float)part[0];
+}
+
+// Sum of `parts` consecutive slabs of n floats into out (split-K partials).
+__global__ void k_fno_sum_parts(const float* __restrict__ x, int parts, int64_t n, float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int p = 0; p < parts; ++p) s += x[(int64_t)p * n + i];
+    out[i] = s;
+  }
+}
+
+// Adam with bias correction on a flat parameter vector.
+__global__ void k_fno_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                           float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// Network input channels of sample b: [Re Y, Im Y, Re X, Im X] / s_b
+// (Y: coarse-step field, X: field before the step), layout [B][4][HW].
+__global__ void k_fno_pack(const float2* __restrict__ Y, const float2* __restrict__ X, const float* __restrict__ inv_s,
+                           float* __restrict__ out, int B, int64_t hw) {
+  const int64_t n = (int64_t)B * hw;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / hw, p = i - b * hw;
+    const float k = inv_s[b];
+    const float2 y = Y[i], x = X[i];
+    float* o = out + b * 4 * hw + p;
+    o[0] = y.x * k;
+    o[hw] = y.y * k;
+    o[2 * hw] = x.x * k;
+    o[3 * hw] = x.y * k;
+  }
+}
+
+// Training target: (F - Y) / s_b as [B][2][HW].
+__global__ void k_fno_residual(const float2* __restrict__ F, const float2* __restrict__ Y,
+                               const float* __restrict__ inv_s, float* __restrict__ tgt, int B, int64_t hw) {
+  const int64_t n = (int64_t)B * hw;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / hw, p = i - b * hw;
+    const float k = inv_s[b];
+    tgt[b * 2 * hw + p] = (F[i].x - Y[i].x) * k;
+    tgt[b * 2 * hw + hw + p] = (F[i].y - Y[i].y) * k;
+  }
+}
+
+// Corrected field: A = Y + s_b (out_re + i out_im).
+__global__ void k_fno_apply(const float2* __restrict__ Y, const float* __restrict__ out, const float* __restrict__ s,
+                            float2* __restrict__ A, int B, int64_t hw) {
+  const int64_t n = (int64_t)B * hw;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / hw, p = i - b * hw;
+    const float k = s[b];
+    A[i] = make_float2(Y[i].x + k * out[b * 2 * hw + p], Y[i].y + k * out[b * 2 * hw + hw + p]);
+  }
+}
+
+// Mean squared error and its gradient: d = 2 (out - tgt) / n; per-block
+// FP64 partial sums of (out - tgt)^2 in part[blockIdx.x].
+__global__ void k_fno_mse(const float* __restrict__ out, const float* __restrict__ tgt, float* __restrict__ d,
+                          int64_t n, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  const float scale = 2.f / (float)n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float e = out[i] - tgt[i];
+    d[i] = scale * e;
+    s += (double)e * e;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace nlse
+
+using namespace nlse;
+
+static unsigned grid1d(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  return (unsigned)(b < 1 ? 1 : (b > 65535 * 16 ? 65535 * 16 : b));
+}
+
+extern "C" {
+
+int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, int64_t sak, int64_t sa,
+                  const float* B, int64_t sbk, int64_t sbn, int64_t sb, float* C, int64_t scm, int64_t scn,
+                  int64_t sc, float alpha, float beta, const float* bias, float* act, void* stream) {
+  if (M < 0 || N < 0 || K < 0 || batch < 1 || batch > 65535 || !A || !B || !C)
+    return fail(NLSE_ERR_ARG, "bad gemm arguments (%d x %d x %d, batch %d)", M, N, K, batch);
+  if (M == 0 || N == 0) return NLSE_OK;
+  GemmArgs g{M, N, K, batch, A, sam, sak, sa, B, sbk, sbn, sb, C, scm, scn, sc, alpha, beta, bias, act};
+  const dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
+  if (grid.y > 65535) return fail(NLSE_ERR_ARG, "gemm M too large (%d)", M);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, (hipStream_t)stream, g);
+  return launch_status("k_gemm_bf16");
+}
+
+int nlse_fno_mix(const float* P1, float* Q, const float* R, int B, int Cin, int Cout, void* stream) {
+  if (!P1 || !Q || !R || B < 1 || Cin < 1 || Cout < 1) return fail(NLSE_ERR_ARG, "bad mix arguments");
+  MixArgs a{P1, Q, R, B, Cin, Cout};
+  hipLaunchKernelGGL(k_fno_mix, dim3(grid1d((int64_t)B * 1024 * Cout)), dim3(256), 0, (hipStream_t)stream, a);
+  return launch_status("k_fno_mix");
+}
+
+int nlse_fno_mix_bwd(const float* P1, const float* dQ, const float* R, float* dP1, float* dR, int B, int Cin,
+                     int Cout, void* stream) {
+  if (!P1 || !dQ || !R || !dP1 || !dR || B < 1 || Cin < 1 || Cout < 1)
+    return fail(NLSE_ERR_ARG, "bad mix_bwd arguments");
+  MixBwdArgs a{P1, dQ, R, dP1, dR, B, Cin, Cout};
+  hipLaunchKernelGGL(k_fno_mix_bwd_input, dim3(grid1d((int64_t)B * 1024 * Cin)), dim3(256), 0,
+                     (hipStream_t)stream, a);
+  int rc = launch_status("k_fno_mix_bwd_input");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fno_mix_bwd_weight, dim3(grid1d((int64_t)1024 * Cout * Cin)), dim3(256), 0,
+                     (hipStream_t)stream, a);
+  return launch_status("k_fno_mix_bwd_weight");
+}
+
+int nlse_fno_gelu_bwd(const float* z, const float* dy, float* dz, int64_t n, void* stream) {
+  if (!z || !dy || !dz || n < 0) return fail(NLSE_ERR_ARG, "bad gelu_bwd arguments");
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_fno_gelu_bwd, dim3(grid1d(n)), dim3(256), 0, (hipStream_t)stream, z, dy, dz, n);
+  return launch_status("k_fno_gelu_bwd");
+}
+
+int nlse_fno_colsum(const float* x, int64_t rows, int64_t cols, int64_t sr, int64_t sc, float* out, void* stream) {
+  if (!x || !out || rows < 0 || cols < 1 || cols > 65535) return fail(NLSE_ERR_ARG, "bad colsum arguments");
+  hipLaunchKernelGGL(k_fno_colsum, dim3((unsigned)cols), dim3(256), 0, (hipStream_t)stream, x, rows, sr, sc, out);
+  return launch_status("k_fno_colsum");
+}
+
+int nlse_fno_sum_parts(const float* x, int parts, int64_t n, float* out, void* stream) {
+  if (!x || !out || parts < 1 || n < 0) return fail(NLSE_ERR_ARG, "bad sum_parts arguments");
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_fno_sum_parts, dim3(grid1d(n)), dim3(256), 0, (hipStream_t)stream, x, parts, n, out);
+  return launch_status("k_fno_sum_parts");
+}
+
+int nlse_fno_pack(const void* Y, const void* X, const float* inv_s, float* out, int B, int64_t hw, void* stream) {
+  if (!Y || !X || !inv_s || !out || B < 1 || hw < 1) return fail(NLSE_ERR_ARG, "bad pack arguments");
+  hipLaunchKernelGGL(k_fno_pack, dim3(grid1d((int64_t)B * hw)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)Y, (const float2*)X, inv_s, out, B, hw);
+  return launch_status("k_fno_pack");
+}
+
+int nlse_fno_residual(const void* F, const void* Y, const float* inv_s, float* tgt, int B, int64_t hw, void* stream) {
+  if (!F || !Y || !inv_s || !tgt || B < 1 || hw < 1) return fail(NLSE_ERR_ARG, "bad residual arguments");
+  hipLaunchKernelGGL(k_fno_residual, dim3(grid1d((int64_t)B * hw)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)F, (const float2*)Y, inv_s, tgt, B, hw);
+  return launch_status("k_fno_residual");
+}
+
+int nlse_fno_apply(const void* Y, const float* out, const float* s, void* A, int B, int64_t hw, void* stream) {
+  if (!Y || !out || !s || !A || B < 1 || hw < 1) return fail(NLSE_ERR_ARG, "bad apply arguments");
+  hipLaunchKernelGGL(k_fno_apply, dim3(grid1d((int64_t)B * hw)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)Y, out, s, (float2*)A, B, hw);
+  return launch_status("k_fno_apply");
+}
+
+int nlse_fno_mse(const float* out, const float* tgt, float* d, int64_t n, double* part, int nparts, void* stream) {
+  if (!out || !tgt || !d || !part || n < 1 || nparts < 1) return fail(NLSE_ERR_ARG, "bad mse arguments");
+  hipLaunchKernelGGL(k_fno_mse, dim3((unsigned)nparts), dim3(256), 0, (hipStream_t)stream, out, tgt, d, n, part);
+  return launch_status("k_fno_mse");
+}
+
+int nlse_fno_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                  float eps, int64_t t, void* stream) {
+  if (!w || !g || !m || !v || n < 0 || t < 1) return fail(NLSE_ERR_ARG, "bad adam arguments");
+  if (n == 0) return NLSE_OK;
+  const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+  hipLaunchKernelGGL(k_fno_adam, dim3(grid1d(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, b1, b2,
+                     eps, c1, c2);
+  return launch_status("k_fno_adam");
+}
+
+}  // extern "C"
