@@ -158,8 +158,19 @@ static bool col_launch_w(const ColArgs& a, hipStream_t s) {
   using K = ColCfg<N, W>;
   if constexpr (K::OK) {
     const int64_t blocks = a.nslabs * (a.NX / W);
-    hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR>), dim3((unsigned)blocks), dim3(K::WT * C::T), 0, s,
-                       a);
+    // the separable 2-D propagator and the real kernel spectrum of the
+    // nonlocal response get their own instances (fewer live registers
+    // than the run-time dispatch over every kind)
+    const bool prop = a.do_prop != 0;
+    if (prop && a.pkind == PROP_SEP64)
+      hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PROP_SEP64>), dim3((unsigned)blocks),
+                         dim3(K::WT * C::T), 0, s, a);
+    else if (prop && a.pkind == PROP_KREAL)
+      hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PROP_KREAL>), dim3((unsigned)blocks),
+                         dim3(K::WT * C::T), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR>), dim3((unsigned)blocks), dim3(K::WT * C::T), 0, s,
+                         a);
     return true;
   } else {
     return false;

@@ -223,7 +223,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
   const int pkind = PK >= 0 ? PK : a.pkind;
   if (a.do_prop) {
     const int comp = (int)(slab % a.ncomp);
-    if (pkind == PROP_SEP64) {
+    if ((PK < 0 || PK == PROP_SEP64) && pkind == PROP_SEP64) {
       double2 px[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) px[g] = a.px[comp * a.NX + c0 + g * WT];
@@ -233,7 +233,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep_c64(v[g][m], px[g], py, a.scale);
       }
-    } else if (PK < 0 && pkind == PROP_SEP3D) {
+    } else if ((PK < 0 || PK == PROP_SEP3D) && pkind == PROP_SEP3D) {
       // column c = x*NT + t of an (NY, NX, NT) field: complex64(px*py)
       // times the complex128 temporal factor, applied in FP64
       double2 px[G], pt[G];
@@ -249,7 +249,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g][m] = prop_sep3d(v[g][m], px[g], py, pt[g], (double)a.scale);
       }
-    } else if (PK < 0 && pkind == PROP_KREAL) {
+    } else if ((PK < 0 || PK == PROP_KREAL) && pkind == PROP_KREAL) {
       // even real spectrum: entry (ky, kx) of the quadrant table at
       // (min(ky, N-ky), min(kx, NX-kx))
       const int hx = a.NX / 2 + 1;
@@ -269,7 +269,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
           v[g][m] = make_float2(v[g][m].x * f, v[g][m].y * f);
         }
       }
-    } else if (PK < 0 && pkind == PROP_C128) {
+    } else if ((PK < 0 || PK == PROP_C128) && pkind == PROP_C128) {
       const double2* pt = a.ptab128 + c0;
 #pragma unroll
       for (int m = 0; m < P; ++m)
@@ -298,7 +298,8 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
     for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
 }
 
-template <int N, int P, int WT, int G, int GR>
+// PK: the propagator kind compiled in (-1: any kind, chosen at run time).
+template <int N, int P, int WT, int G, int GR, int PK = -1>
 __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   __shared__ float2 lds[col_lds_size<N, WT, GR>()];
   // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   // by neighbouring tiles on the same XCD (one L2 fill instead of two)
   const int64_t nblk = (int64_t)gridDim.x;
   const int64_t b = (nblk % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
-  col_body<N, P, WT, G, GR>(a, b, lds);
+  col_body<N, P, WT, G, GR, PK>(a, b, lds);
 }
 
 // ----------------------------------------------------------------- 1-D
