@@ -60,7 +60,7 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float) -> dict:
+def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float, mode: str = "k0") -> dict:
     """Oracle (complex64, scipy single-precision FFT, elementwise steps on a
     thread pool, scipy oaconvolve for the nonlocal term) on the host cores:
     steps of the same 2048^2 workload until `budget_s` of wall time has
@@ -70,6 +70,7 @@ def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float) -> dic
 
     sim = W.config1(oracle.NLSE, N, nonlocal_length=nl_length, fft="pocketfft32", workers=threads,
                     threads=threads)
+    sim.nonlocal_mode = mode
     E0 = W.config1_field(sim)
     A, A_sq = sim._prepare_output_array(E0, True)
     prop = sim._build_propagator()
@@ -88,20 +89,29 @@ def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float) -> dic
         "cores": threads,
         "kind": "port",
         "sample": f"{n} split steps of the {N}x{N} config-1 workload "
-                  f"({'nonlocal, scipy oaconvolve' if nl_length > 0 else 'without the nonlocal term'}) on the "
+                  f"({('nonlocal, ' + ('FP64 FFT Lorentzian' if mode == 'lorentzian' else 'scipy oaconvolve')) if nl_length > 0 else 'without the nonlocal term'}) on the "
                   f"CPU oracle: complex64, scipy.fft single precision with {threads} workers, elementwise "
                   f"steps on {threads} threads; {el:.1f} s; host CPU: {cpu_model()} "
                   f"(os.cpu_count()={os.cpu_count()})",
     }
 
 
-def kernel_bytes(P: int, N: int, r: int, local: bool) -> dict:
+def kernel_bytes(P: int, N: int, r: int, local: bool, lorentzian: bool = False) -> dict:
     """Algorithmic HBM bytes per launch of each kernel of the step (P field
     points, complex64 field, float32 V).  Nonlocal one-component fields use
     the packed intensity torus of P/2 complex points (4 B per field point)
-    and band points within r of its edges."""
+    and band points within r of its edges; the Fourier-space (Lorentzian)
+    response keeps one complex intensity line per row (8 B per point)."""
     if local:
         return {"row": (0, P * (16 + 4)), "col": (1, P * 16)}
+    if lorentzian:
+        khat = 4 * (N // 2 + 1) * (N // 2 + 1)
+        return {
+            "col": (1, P * 16),
+            "nlc_z": (2, P * (8 + 8 + 8)),
+            "colZ": (4, P * 16 + khat),
+            "nlc_nl": (5, P * (8 + 8 + 4 + 8)),
+        }
     band = 4 * r * N + (N - 4 * r) * 2 * r
     khat = 4 * (N // 4 + 1) * (N // 2 + 1)
     return {
@@ -123,6 +133,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--local", action="store_true", help="drop the nonlocal term (config-1 physics otherwise)")
+    ap.add_argument("--nonlocal-mode", choices=("k0", "lorentzian"), default="k0",
+                    help="k0: the reference's K0 convolution; lorentzian: 1/(1+(k l)^2) in Fourier space")
     ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -153,6 +165,9 @@ def bench_c1(args) -> int:
     nl_length = 0.0 if args.local else 20e-6
     sim = W.config1(NLSE, N, nonlocal_length=nl_length)
     sim.col_width = args.col_width
+    lorentzian = not args.local and args.nonlocal_mode == "lorentzian"
+    if lorentzian:
+        sim.nonlocal_mode = "lorentzian"
     E0 = W.config1_field(sim)
     sim._own_propagator()
     sim._send_arrays_to_gpu()
@@ -161,7 +176,7 @@ def bench_c1(args) -> int:
     sim._configure(eng, A, sim.V, sim.propagator, "single")
     P = A.numel()
     r = 0 if args.local else int(tuple(sim.nl_profile.shape)[0] // 2)
-    kb = kernel_bytes(P, N, r, args.local)
+    kb = kernel_bytes(P, N, r, args.local, lorentzian)
     step_bytes = sum(b for _, b in kb.values())
 
     # warm-up, then K steps bracketed by synchronisation
@@ -201,6 +216,9 @@ def bench_c1(args) -> int:
                "precision='single'")
     if args.local:
         workload = f"config-1 physics without the nonlocal term: {N}x{N} complex64 NLSE split step, " + physics
+    elif lorentzian:
+        workload = (f"config-1: {N}x{N} complex64 NLSE split step with the nonlocal response applied in "
+                    "Fourier space as 1/(1+(k l)^2), l = 20 um (periodic), " + physics)
     else:
         workload = (f"config-1: {N}x{N} complex64 NLSE split step with the nonlocal response "
                     f"(nl_length 20 um, reference K0 kernel {2 * r + 1}x{2 * r + 1}, zero-padded linear "
@@ -221,7 +239,7 @@ def bench_c1(args) -> int:
         "config": {
             "workload": workload,
             "grid": [N, N],
-            "nonlocal_kernel": None if args.local else [2 * r + 1, 2 * r + 1],
+            "nonlocal_kernel": None if args.local else ("lorentzian" if lorentzian else [2 * r + 1, 2 * r + 1]),
             "kernels": {"row_lines_per_block": row_lines, "col_width": col_width},
         },
         "roofline": {
@@ -231,7 +249,7 @@ def bench_c1(args) -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": measured_traffic(dom, N, args.local),
+            "traffic": None if lorentzian else measured_traffic(dom, N, args.local),
         },
         "kernels": kern,
         "step_roofline": {
@@ -247,7 +265,8 @@ def bench_c1(args) -> int:
     }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads, nl_length)
+        result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads, nl_length,
+                                              "lorentzian" if lorentzian else "k0")
     print(json.dumps(result), flush=True)
     return 0
 

@@ -189,6 +189,13 @@ int nlse_plan_set_drive(nlse_plan* plan, const void* Fr0_dev, const void* Fr1_de
  * profile is).  NULL disables.  3-D plans reject it (the reference's
  * oaconvolve rejects a 2-D kernel on a 3-D field). */
 int nlse_plan_set_nonlocal(nlse_plan* plan, const double* kernel, int64_t kh, int64_t kw);
+/* Nonlocal response given in Fourier space (no counterpart in the
+ * reference, whose response is the real-space K0 convolution above): every
+ * nonlinear application uses the periodic convolution of |A|^2 whose
+ * spectrum is the real, even function `quad` (host float64, the quadrant
+ * [ky = 0..ny/2][kx = 0..nx/2], e.g. the Lorentzian 1/(1 + k^2 l^2)).
+ * 2-D plans only; NULL disables. */
+int nlse_plan_set_nonlocal_spectrum(nlse_plan* plan, const double* quad, int64_t qy, int64_t qx);
 /* Run nsteps split steps on A (device, real space in and out).
  * precision: 0 "single" (Lie), 1 "double" (Strang).  step0 is the absolute
  * index of the first step (drive time index). */

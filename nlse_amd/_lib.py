@@ -86,6 +86,7 @@ SIGNATURES = {
     "nlse_plan_set_potential": (c_int, [c_void_p, c_void_p, c_int, c_i64]),
     "nlse_plan_set_drive": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_double]),
     "nlse_plan_set_nonlocal": (c_int, [c_void_p, c_void_p, c_i64, c_i64]),
+    "nlse_plan_set_nonlocal_spectrum": (c_int, [c_void_p, c_void_p, c_i64, c_i64]),
     "nlse_run": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),
     "nlse_run_pass": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_int, c_void_p]),
     "nlse_run_graph": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_i64, c_void_p]),

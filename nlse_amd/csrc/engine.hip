@@ -459,7 +459,8 @@ struct nlse_plan {
   struct {
     bool on = false;
     int r = 0;                      // kernel half-width (after trimming)
-    int64_t M = 0;                  // torus rows of the packed intensity
+    int64_t M = 0;                  // row pairing of the packed intensity (torus rows unless full)
+    bool full = false;              // one component on the full NY-row torus (spectral response)
     bool direct = false;            // every point convolved directly
     const LineOps* zops = nullptr;  // columns of length M
     const float4* twz = nullptr;
@@ -813,6 +814,43 @@ static int nlc_setup_2d(nlse_plan* p, const double* kernel, int kw) {
   return NLSE_OK;
 }
 
+int nlse_plan_set_nonlocal_spectrum(nlse_plan* p, const double* quad, int64_t qy, int64_t qx) {
+  if (!p) return fail(NLSE_ERR_ARG, "null plan");
+  graph_free(p);  // captured launches bake in table pointers
+  nlc_free(p);
+  if (!quad) return NLSE_OK;
+  if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "spectral nonlocal responses are for 2-D plans");
+  const int64_t ny = p->d.ny, nx = p->d.nx, items = p->d.items;
+  if (qy != ny / 2 + 1 || qx != nx / 2 + 1)
+    return fail(NLSE_ERR_ARG, "spectrum quadrant must be (%lld, %lld)", (long long)(ny / 2 + 1),
+                (long long)(nx / 2 + 1));
+  auto& n = p->nlc;
+  int rc = NLSE_OK;
+  n.r = 0;
+  n.direct = false;
+  n.full = p->d.ncomp == 1;
+  n.M = p->d.ncomp == 1 ? ny / 2 : ny;
+  n.nb = 0;
+  std::vector<int> rowmap(ny, -1);
+  std::vector<float> kq((size_t)qy * qx);
+  const double norm = 1.0 / ((double)ny * (double)nx);
+  for (size_t i = 0; i < kq.size(); ++i) kq[i] = (float)(quad[i] * norm);
+  n.zops = line_ops(ny);
+  if (!n.zops) rc = fail(NLSE_ERR_ARG, "unsupported column length %lld", (long long)ny);
+  if (!rc) rc = n.zops->twiddles(&n.twz);
+  if (!rc) rc = upload((void**)&n.rowmap, rowmap.data(), rowmap.size() * sizeof(int));
+  if (!rc) rc = upload((void**)&n.kq, kq.data(), kq.size() * sizeof(float));
+  if (!rc && hipMalloc((void**)&n.Z, sizeof(float2) * items * ny * nx) != hipSuccess)
+    rc = fail(NLSE_ERR_HIP, "hipMalloc of the intensity buffer failed");
+  if (rc) {
+    nlc_free(p);
+    return rc;
+  }
+  n.colw = n.zops->colw(nx, p->d.col_width);
+  n.on = true;
+  return NLSE_OK;
+}
+
 int nlse_plan_set_nonlocal(nlse_plan* p, const double* kernel, int64_t kh, int64_t kw) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
   graph_free(p);  // captured launches bake in table pointers
@@ -923,6 +961,7 @@ static NlcArgs plan_nlc_args(const nlse_plan* p, void* A) {
   a.nb = n.nb;
   a.r = n.r;
   a.direct = n.direct ? 1 : 0;
+  a.full = n.full ? 1 : 0;
   return a;
 }
 

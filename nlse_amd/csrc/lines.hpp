@@ -143,8 +143,13 @@ static int nlc_launch(int stage, const NlcArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
   constexpr int L = C::LPB2;
   const int64_t blocks = (a.ngroups + L - 1) / L;
-  if (stage == 1)
+  const bool full = a.full && a.ncomp == 1;
+  if (stage == 1 && full)
+    hipLaunchKernelGGL((k_nlc_z<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  else if (stage == 1)
     hipLaunchKernelGGL((k_nlc_z<N, C::P, L>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  else if (full)
+    hipLaunchKernelGGL((k_nlc_nl<N, C::P, L, 1, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
   else if (a.ncomp == 1)
     hipLaunchKernelGGL((k_nlc_nl<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
   else

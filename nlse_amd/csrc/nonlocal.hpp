@@ -60,6 +60,10 @@ struct NlcArgs {
   int nb;
   int r;
   int direct;
+  // one component on the full NY x NX torus (a spectral response given in
+  // Fourier space, no band points): the two rows of a pair keep separate
+  // intensity rows Z[item][y] = I(y) + 0i, NY per item
+  int full;
 };
 
 // Geometry of line group gi: the two field lines it owns and their slabs.
@@ -92,7 +96,7 @@ __device__ __forceinline__ NlcLines nlc_lines(const NlcArgs& a, int64_t gi, int 
 
 // Stage 1: [I_x] of the two field lines (stored back in real space), then
 // Z = |A_a|^2 + i |A_b|^2 and its forward transform along x.
-template <int N, int P, int LPB>
+template <int N, int P, int LPB, int FULL = 0>
 __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_z(NlcArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
@@ -125,7 +129,20 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_z(NlcArgs a) {
         for (int m = 0; m < P; ++m) a.A[L.off[g] + t + m * T] = v[g][m];
     }
   }
-  if (a.write_z) {
+  if constexpr (FULL) {
+    // one real intensity per complex line, each row on its own
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(sqmag(v[g][m]), 0.f);
+    fft_line<N, P, 2, false, 1>(v, t, a.tw, lline, LINE);
+    if (active) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int m = 0; m < P; ++m) a.Z[(L.item * a.ny + L.y[g]) * N + t + m * T] = v[g][m];
+    }
+  } else if (a.write_z) {
     float2 z[1][P];
 #pragma unroll
     for (int m = 0; m < P; ++m) z[0][m] = make_float2(sqmag(v[0][m]), sqmag(v[1][m]));
@@ -153,7 +170,7 @@ __device__ __forceinline__ float nlc_band(const NlcArgs& a, int bi, int64_t slab
 // one after the other (potential, nonlinear step, F_x, store), so only one
 // row's working set is live at a time; two components (NC = 2) couple at
 // every point and are transformed one line after the other.
-template <int N, int P, int LPB, int NC>
+template <int N, int P, int LPB, int NC, int FULL = 0>
 __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
@@ -171,8 +188,9 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
 #pragma unroll
   for (int m = 0; m < P; ++m) z[0][m] = make_float2(0.f, 0.f);
   if (!a.direct) {
+    const int64_t zrow = FULL ? L.item * a.ny + L.y[0] : gs;
 #pragma unroll
-    for (int m = 0; m < P; ++m) z[0][m] = active ? a.Z[gs * N + t + m * T] : make_float2(0.f, 0.f);
+    for (int m = 0; m < P; ++m) z[0][m] = active ? a.Z[zrow * N + t + m * T] : make_float2(0.f, 0.f);
   }
   float2 v[NC][P];
 #pragma unroll
@@ -225,6 +243,17 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
   if constexpr (NC == 1) {
     const NLParams1 p = reinterpret_cast<const NLParams1*>(a.nlp)[L.item];
     float I[P];
+    if constexpr (FULL) {
+      // the second row's intensity line, transformed on its own; the first
+      // row's goes to z.x (its imaginary part is zero)
+      float2 z2[1][P];
+#pragma unroll
+      for (int m = 0; m < P; ++m)
+        z2[0][m] = active ? a.Z[(L.item * a.ny + L.y[1]) * N + t + m * T] : make_float2(0.f, 0.f);
+      fft_line<N, P, 1, true>(z2, t, a.tw, lline, LINE);
+#pragma unroll
+      for (int m = 0; m < P; ++m) z[0][m].y = z2[0][m].x;
+    }
 #pragma unroll
     for (int m = 0; m < P; ++m) I[m] = z[0][m].x;
     load_row_v(L.y[0]);

@@ -176,6 +176,21 @@ class Plan:
         _lib.check(lib.nlse_plan_set_nonlocal(self._h, k.ctypes.data, k.shape[0], k.shape[1]), "set_nonlocal")
         self._last["nonlocal"] = key
 
+    def set_nonlocal_spectrum(self, quad) -> None:
+        """Nonlocal response given in Fourier space: real even spectrum as
+        the quadrant [ny/2 + 1][nx/2 + 1] (host), or None."""
+        lib = _lib.load()
+        if quad is None:
+            self.set_nonlocal(None)
+            return
+        q = np.ascontiguousarray(np.asarray(quad, dtype=np.float64))
+        key = ("spectrum", q.shape, q.tobytes())
+        if self._last.get("nonlocal") == key:
+            return
+        _lib.check(lib.nlse_plan_set_nonlocal_spectrum(self._h, q.ctypes.data, q.shape[0], q.shape[1]),
+                   "set_nonlocal_spectrum")
+        self._last["nonlocal"] = key
+
     # -- execution -------------------------------------------------------
     def run(self, A: torch.Tensor, nsteps: int, precision: int = 0, step0: int = 0) -> None:
         if A.dtype != torch.complex64 or not A.is_contiguous() or not A.is_cuda:

@@ -374,12 +374,25 @@ class NLSE:
 
     def _configure_nonlocal(self, eng: Plan) -> None:
         """nlse.py:406-409,444-447: with nl_length > 0 every nonlinear
-        application sees oaconvolve(|A|^2, nl_profile, mode="same")."""
-        if self.nl_length > 0:
+        application sees oaconvolve(|A|^2, nl_profile, mode="same").
+        With the attribute ``nonlocal_mode = "lorentzian"`` (no reference
+        counterpart) the response is instead the periodic convolution whose
+        spectrum is 1/(1 + |k|^2 nl_length^2), applied in Fourier space."""
+        if self.nl_length > 0 and getattr(self, "nonlocal_mode", "k0") == "lorentzian":
+            eng.set_nonlocal_spectrum(self._lorentzian_quadrant())
+        elif self.nl_length > 0:
             eng.set_nonlocal(D.to_host(self.nl_profile) if D.is_device_tensor(self.nl_profile)
                              else np.asarray(self.nl_profile))
         else:
             eng.set_nonlocal(None)
+
+    def _lorentzian_quadrant(self) -> np.ndarray:
+        """1/(1 + (kx^2 + ky^2) l^2) for ky = 0..NY/2, kx = 0..NX/2 (FP64)."""
+        if len(self._last_axes) != 2:
+            raise NotImplementedError("the Lorentzian response is implemented for 2-D fields")
+        kx = np.asarray(self.Kx, np.float64)[: self.NX // 2 + 1]
+        ky = np.asarray(self.Ky, np.float64)[: self.NY // 2 + 1]
+        return 1.0 / (1.0 + (kx[None, :] ** 2 + ky[:, None] ** 2) * self.nl_length**2)
 
     def _configure_tables(self, eng: Plan, V, propagator) -> None:
         pf = self._prop_factors

@@ -206,7 +206,15 @@ class NLSE:
 
     def _convolution(self, A_sq: np.ndarray) -> np.ndarray:
         """oaconvolve(A_sq, nl_profile, mode='same', axes=last_axes)
-        (nlse.py:186,406-409); FP64 result like scipy's float32 x float64."""
+        (nlse.py:186,406-409); FP64 result like scipy's float32 x float64.
+        nonlocal_mode = "lorentzian" (no reference counterpart): the
+        periodic convolution with spectrum 1/(1 + |k|^2 l^2), FP64 FFTs."""
+        if getattr(self, "nonlocal_mode", "k0") == "lorentzian":
+            Kx = 2 * np.pi * np.fft.fftfreq(self.NX, d=self.delta_X)
+            Ky = 2 * np.pi * np.fft.fftfreq(self.NY, d=self.delta_Y)
+            spec = 1.0 / (1.0 + (Kx[None, :] ** 2 + Ky[:, None] ** 2) * self.nl_length**2)
+            return np.real(np.fft.ifft2(np.fft.fft2(np.asarray(A_sq, np.float64), axes=(-2, -1)) * spec,
+                                        axes=(-2, -1)))
         kern = self.nl_profile
         while kern.ndim < A_sq.ndim:
             kern = kern[np.newaxis]

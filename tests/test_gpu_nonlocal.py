@@ -270,3 +270,38 @@ def test_nlse_nonlocal_2048_parity(gpu):
     err = rel_l2(out, o.out_field(E0, 1e-4))
     print(f"NLSE nonlocal 2048^2 K=79x79 10 steps: {err:.2e}")
     assert err < PARITY
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("ny,nx", [(128, 128), (64, 128)])
+def test_nlse_lorentzian_matches_oracle(gpu, precision, ny, nx):
+    """nonlocal_mode = "lorentzian": the response applied in Fourier space as
+    1/(1 + k^2 l^2) (config-1 text; no reference counterpart), one
+    component on the full torus."""
+    g, o = _pair("NLSE", ny, nx, 3.5)
+    for s in (g, o):
+        s.delta_z = 1e-5
+        s.nonlocal_mode = "lorentzian"
+        s.V = (-1e-4 * np.exp(-(s.XX**2 + s.YY**2) / (70e-6) ** 2)).astype(np.float32)
+    G = np.exp(-(g.XX.astype(np.float64) ** 2 + g.YY.astype(np.float64) ** 2) / waist**2)
+    E0 = (G * _noise(G.shape, 17)).astype(np.complex64)
+    out = g.out_field(E0, 1e-3, precision=precision, verbose=False)
+    ref = o.out_field(E0, 1e-3, precision=precision)
+    err = rel_l2(out, ref)
+    print(f"NLSE Lorentzian {ny}x{nx} {precision}: {err:.2e}")
+    assert err < PARITY
+
+
+def test_cnlse_lorentzian_matches_oracle(gpu):
+    g, o = _pair("CNLSE", 128, 128, 3.5, n12=-1e-10, omega=2e3)
+    for s in (g, o):
+        s.delta_z = 1e-5
+        s.nonlocal_mode = "lorentzian"
+        s.power2 = 0.3
+        s.n22 = -4e-10
+    G = np.exp(-(g.XX.astype(np.float64) ** 2 + g.YY.astype(np.float64) ** 2) / waist**2)
+    E0 = np.stack([G * _noise(G.shape, 18), G * _noise(G.shape, 19) * 0.5]).astype(np.complex64)
+    out = g.out_field(E0, 1e-3, verbose=False)
+    err = rel_l2(out, o.out_field(E0, 1e-3))
+    print(f"CNLSE Lorentzian: {err:.2e}")
+    assert err < PARITY

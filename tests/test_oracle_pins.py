@@ -314,3 +314,19 @@ def test_z_beyond_L_zeroes_n2():
     lin.n2 = 0.0
     lin.delta_z = 1e-5
     assert np.allclose(out, lin.out_field(E0, 2e-4), rtol=1e-6, atol=1e-6 * np.abs(out).max())
+
+
+def test_lorentzian_response_closed_form():
+    """The oracle's Fourier-space Lorentzian response maps 1 + cos(k0 x) to
+    1 + cos(k0 x) / (1 + k0^2 l^2) (a single mode is an eigenfunction)."""
+    import numpy as np
+
+    import oracle
+
+    o = oracle.NLSE(0, 1.0, 4e-3, -1e-9, None, 1e-3, NX=64, NY=32, nl_length=30e-6)
+    o.nonlocal_mode = "lorentzian"
+    k0 = 2 * np.pi * 5 / (64 * o.delta_X)
+    I = 1 + np.cos(k0 * np.arange(64) * o.delta_X)[None, :] * np.ones((32, 1))
+    out = o._convolution(I)
+    expect = 1 + np.cos(k0 * np.arange(64) * o.delta_X)[None, :] / (1 + k0**2 * 30e-6**2)
+    assert np.abs(out - expect).max() < 1e-12
