@@ -118,6 +118,60 @@ int nlse_cavity_drive(void* A_dev, int64_t n_items, int64_t item_size, int64_t i
  * (power integrals of nlse.py:297-303 and callbacks.py:54). */
 int nlse_sumsq(const void* A_dev, int64_t n_items, int64_t item_size, double* out_dev, void* stream);
 
+/* Nonlinear index change of callbacks.evaluate_delta_n (callbacks.py:57-82)
+ * and callbacks.adapt_delta_z (callbacks.py:85-112) in FP32, rounded op by
+ * op in the reference's evaluation order:
+ *   q = re*re + im*im                    (qc == 0, callbacks.py:80)
+ *   q = re*re + ((im*im)*qc)*qe / 2      (qc = c, qe = eps0: the operator
+ *                                         precedence of callbacks.py:109)
+ *   delta_n = (coef[item] * q) / (1 + q / isat[item])
+ * nlse_delta_n writes delta_n for every point to out_dev (float32, A's
+ * shape); nlse_delta_n_max writes its maximum over all items and points
+ * to out_dev[0] (the `delta_n.max()` of callbacks.py:111). */
+int nlse_delta_n(const void* A_dev, int64_t n_items, int64_t item_size, float qc, float qe, nlse_bparam coef,
+                 nlse_bparam isat, float* out_dev, void* stream);
+int nlse_delta_n_max(const void* A_dev, int64_t n_items, int64_t item_size, float qc, float qe,
+                     nlse_bparam coef, nlse_bparam isat, float* out_dev, void* stream);
+
+/* ------------------------------------------- initial state and drive
+ * Number kinds of nlse_vortex's arrays. */
+#define NLSE_NUM_F32 0
+#define NLSE_NUM_F64 1
+#define NLSE_NUM_I32 2
+#define NLSE_NUM_I64 3
+
+/* Vortex phase imprint of kernels_gpu.vortex_cp (kernels_gpu.py:159-176):
+ *   im[k] += angle(((ii[k] - i0) + 1j*(jj[k] - j0)) ** charge)
+ * for k < n, evaluated as the principal value of charge*atan2 in (-pi, pi]
+ * (FP64), added in im's precision.  im is float32 or float64; ii, jj (the
+ * meshgrid coordinates) any NLSE_NUM_* kind.  The CPU kernel of the
+ * reference (kernels_cpu.py:164-183) shadows i, j with its loop indices;
+ * this follows the GPU kernel. */
+int nlse_vortex(void* im_dev, int im_kind, int64_t n, double i0, double j0, const void* ii_dev,
+                const void* jj_dev, int coord_kind, int charge, void* stream);
+
+/* Counter-based random numbers (no counterpart in the reference, which
+ * draws from cupy.random / numpy.random): Philox4x64-10, the generator of
+ * numpy.random.Philox.  Block b (b < nblocks) encrypts the counter
+ * (counter + b + 1) with key[2] and writes its four words to out_dev[4b..];
+ * with the same key and counter the words equal numpy.random.Philox(
+ * counter=counter, key=key).random_raw(4 * nblocks). */
+int nlse_philox4x64(uint64_t* out_dev, int64_t nblocks, const uint64_t counter[4], const uint64_t key[2],
+                    void* stream);
+
+/* DDGPE.add_noise (ddgpe.py:96-125) on a (n_items, 2, item_size) field:
+ *   A[:, 0] += amp0 * dz * (z0 + 1j*z1),  A[:, 1] += amp1 * dz * (z2 + 1j*z3)
+ * with amp_c = noise * sqrt(gamma_c / (4 dX dY)).  Point p takes Philox block
+ * p of (counter, key) (as nlse_philox4x64) and turns words (0,1) and (2,3)
+ * into standard normals by Box-Muller on 53-bit uniforms u:
+ *   z0 = sqrt(-2 ln(1 - u0)) cos(2 pi u1), z1 = ... sin(2 pi u1).
+ * The same draw is added to every item (the reference draws one (NY, NX)
+ * array and broadcasts it).  Complex128 arithmetic, complex64 result.
+ * Bitwise equality with cupy.random.normal is not attainable; successive
+ * calls must advance the counter (e.g. counter[1] = call index). */
+int nlse_add_noise(void* A_dev, int64_t n_items, int64_t item_size, const uint64_t counter[4],
+                   const uint64_t key[2], double dz, double amp0, double amp1, void* stream);
+
 /* ------------------------------------------------------ fused engine
  * A plan runs whole split-step loops (nlse.py:566-588) on one field shape.
  * Its tables are set once; nlse_run then performs `nsteps` steps from real

@@ -1,16 +1,15 @@
 """Generic callbacks (reference NLSE/callbacks.py:1-112) for device fields.
 
 Callbacks are called as ``cb(simu, A, z, i, *args)`` after every step, with
-``A`` the device field (torch tensor).  Reductions run in FP64 on the device
-(libnlse_hip.so ``nlse_sumsq``).
+``A`` the device field (torch tensor).  Power sums run in FP64 on the device
+(libnlse_hip.so ``nlse_sumsq``); the nonlinear index change and its maximum
+in FP32 on the device (``nlse_delta_n``, ``nlse_delta_n_max``).
 """
 
 from __future__ import annotations
 
 import numpy as np
 import torch
-from scipy.constants import c, epsilon_0
-
 from . import kernels_hip
 
 
@@ -33,27 +32,36 @@ def norm(simu, A, z: float, i: int, save_every: int, norms) -> None:
 
 def evaluate_delta_n(simu, A, z: float, i: int, save_every: int, delta_n) -> None:
     """Nonlinear index change c eps0/2 n2 |A|^2 / (1 + |A|^2/I_sat)
-    (callbacks.py:57-82); stored on the host."""
+    (callbacks.py:57-82), computed on the device in FP32 (nlse_delta_n);
+    stored on the device when delta_n is a device tensor, else on the host."""
     if i % save_every == 0:
-        A_sq = (A.real * A.real + A.imag * A.imag).detach().cpu().numpy()
-        n2 = np.asarray(simu.n2.detach().cpu().numpy() if isinstance(simu.n2, torch.Tensor) else simu.n2)
-        isat = np.asarray(simu.I_sat.detach().cpu().numpy() if isinstance(simu.I_sat, torch.Tensor)
-                          else simu.I_sat)
-        delta_n[i // save_every] = c * epsilon_0 / 2 * n2 * A_sq / (1 + A_sq / isat)
+        dn = kernels_hip.delta_n(A, simu.n2, simu.I_sat, tdims=len(simu._last_axes))
+        if isinstance(delta_n, torch.Tensor):
+            delta_n[i // save_every] = dn
+        else:
+            delta_n[i // save_every] = dn.cpu().numpy()
+
+
+def adapt_update(simu, A) -> None:
+    """The update of callbacks.py:107-112: delta_z = |1/(k max delta_n)|/12
+    with delta_n = |n2| q/(1 + q/I_sat) and q = re^2 + im^2 c eps0/2 (the
+    reference's operator precedence).  The maximum is reduced on the device
+    (nlse_delta_n_max); one float is read back.  The cached propagator is
+    not rebuilt, as in the reference, unless simu.adapt_rebuilds_propagator
+    is set (no reference counterpart)."""
+    m = kernels_hip.delta_n_max(A, simu.n2, simu.I_sat, tdims=len(simu._last_axes))
+    with np.errstate(divide="ignore"):
+        z_nl = float(1 / (simu.k * np.float64(m)))
+    simu.delta_z = np.abs(z_nl) / 12
+    if getattr(simu, "adapt_rebuilds_propagator", False):
+        simu._own_propagator()
 
 
 def adapt_delta_z(simu, A, z: float, i: int, update_every: int, delta_z: list) -> None:
-    """Step-size update (callbacks.py:85-112), including the reference's
-    precedence (only im^2 is scaled by c eps0/2, callbacks.py:109).  The
-    cached propagator is not rebuilt, as in the reference."""
+    """Step-size update (callbacks.py:85-112): records the step size of every
+    step and updates it every update_every steps.  When it is the only
+    callback of NLSE.out_field the loop runs the steps between two updates
+    as one engine call (same step sizes, same z bookkeeping)."""
     delta_z.append(simu.delta_z)
     if i % update_every == 0:
-        re = A.real.detach().cpu().numpy().astype(np.float64)
-        im = A.imag.detach().cpu().numpy().astype(np.float64)
-        A_sq = re * re + im * im * c * epsilon_0 / 2
-        n2 = np.asarray(simu.n2.detach().cpu().numpy() if isinstance(simu.n2, torch.Tensor) else simu.n2)
-        isat = np.asarray(simu.I_sat.detach().cpu().numpy() if isinstance(simu.I_sat, torch.Tensor)
-                          else simu.I_sat)
-        delta_n = np.abs(n2) * A_sq / (1 + A_sq / isat)
-        z_nl = float(1 / (simu.k * delta_n.max()))
-        simu.delta_z = np.abs(z_nl) / 12
+        adapt_update(simu, A)

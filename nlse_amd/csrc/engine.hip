@@ -322,6 +322,68 @@ int scratch_doubles(size_t n, double** out) {
   return NLSE_OK;
 }
 
+// Nonlinear index change of callbacks.py:57-112 in FP32, rounded op by op in
+// the order the reference's array expressions evaluate:
+//   q = re*re + im*im                       (evaluate_delta_n, callbacks.py:80)
+//   q = re*re + ((im*im)*qc)*qe/2           (adapt_delta_z, callbacks.py:109,
+//                                            qc = c, qe = eps0; qc = 0 selects
+//                                            the first form)
+//   delta_n = (coef*q) / (1 + q/I_sat)
+// coef is c eps0/2 n2 (evaluate) or |n2| (adapt).
+__device__ __forceinline__ float delta_n_point(float2 a, float qc, float qe, float coef, float isat) {
+  // each product rounds before the sum, as numpy's array ops do (plain
+  // operators here: the contract(off) pragma does not reach into the bodies
+  // of __fmul_rn / __fadd_rn)
+#pragma clang fp contract(off)
+  const float yy = a.y * a.y;
+  const float im = qc != 0.f ? ((yy * qc) * qe) * 0.5f : yy;
+  const float q = a.x * a.x + im;
+  return (coef * q) / (1.0f + q / isat);
+}
+
+__global__ void k_delta_n(const float2* A, int64_t items, int64_t isz, float qc, float qe, nlse_bparam coef,
+                          nlse_bparam isat, float* out) {
+  const int64_t n = items * isz;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t item = i / isz;
+    out[i] = delta_n_point(A[i], qc, qe, (float)bp(coef, item), (float)bp(isat, item));
+  }
+}
+
+// max over all points: block maxima (as doubles, exact) then one block
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_delta_n_max_part(const float2* A, int64_t isz, int64_t nchunks,
+                                                                   float qc, float qe, nlse_bparam coef,
+                                                                   nlse_bparam isat,
+                                                                   double* part) {
+  const int64_t item = blockIdx.x / nchunks, ch = blockIdx.x % nchunks;
+  const float2* a = A + item * isz;
+  const float cf = (float)bp(coef, item), is = (float)bp(isat, item);
+  const int64_t beg = ch * SUMSQ_CHUNK, end = beg + SUMSQ_CHUNK < isz ? beg + SUMSQ_CHUNK : isz;
+  float m = -__builtin_inff();
+  for (int64_t i = beg + threadIdx.x; i < end; i += SUMSQ_THREADS) m = fmaxf(m, delta_n_point(a[i], qc, qe, cf, is));
+  __shared__ float red[SUMSQ_THREADS];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = SUMSQ_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_max_final(const double* part, int64_t n, float* out) {
+  float m = -__builtin_inff();
+  for (int64_t i = threadIdx.x; i < n; i += SUMSQ_THREADS) m = fmaxf(m, (float)part[i]);
+  __shared__ float red[SUMSQ_THREADS];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = SUMSQ_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 unsigned grid_for(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b > 65536) b = 65536;
@@ -422,6 +484,34 @@ int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* str
   if (rc) return rc;
   hipLaunchKernelGGL(k_sumsq_final, dim3((unsigned)items), dim3(SUMSQ_THREADS), 0, s, part, nchunks, out);
   return launch_status("k_sumsq_final");
+}
+
+int nlse_delta_n(const void* A, int64_t items, int64_t isz, float qc, float qe, nlse_bparam coef,
+                 nlse_bparam isat, float* out, void* stream) {
+  if (!A || !out || items < 0 || isz < 0 || !bparam_ok(coef) || !bparam_ok(isat))
+    return fail(NLSE_ERR_ARG, "bad delta_n arguments");
+  const int64_t n = items * isz;
+  if (n == 0) return NLSE_OK;
+  hipLaunchKernelGGL(k_delta_n, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const float2*)A, items, isz,
+                     qc, qe, coef, isat, out);
+  return launch_status("k_delta_n");
+}
+
+int nlse_delta_n_max(const void* A, int64_t items, int64_t isz, float qc, float qe, nlse_bparam coef,
+                     nlse_bparam isat, float* out, void* stream) {
+  if (!A || !out || items < 1 || isz < 1 || !bparam_ok(coef) || !bparam_ok(isat))
+    return fail(NLSE_ERR_ARG, "bad delta_n_max arguments");
+  const int64_t nchunks = (isz + SUMSQ_CHUNK - 1) / SUMSQ_CHUNK;
+  double* part = nullptr;
+  int rc = scratch_doubles((size_t)(items * nchunks), &part);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_delta_n_max_part, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s,
+                     (const float2*)A, isz, nchunks, qc, qe, coef, isat, part);
+  rc = launch_status("k_delta_n_max_part");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(SUMSQ_THREADS), 0, s, part, items * nchunks, out);
+  return launch_status("k_max_final");
 }
 
 // ------------------------------------------------------- fused plan
@@ -567,7 +657,6 @@ static int upload(void** dst, const void* src, size_t bytes) {
 
 int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* py) {
   if (!p || !px || !py) return fail(NLSE_ERR_ARG, "null argument");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim == 1) return fail(NLSE_ERR_ARG, "1-D plans take nlse_plan_set_propagator_c128");
   const int nc = p->d.ncomp;
   int rc = upload((void**)&p->px, px, sizeof(double2) * nc * p->d.nx);
@@ -579,7 +668,6 @@ int nlse_plan_set_propagator_sep(nlse_plan* p, const double* px, const double* p
 
 int nlse_plan_set_propagator_c128(nlse_plan* p, const double* prop) {
   if (!p || !prop) return fail(NLSE_ERR_ARG, "null argument");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 1) return fail(NLSE_ERR_ARG, "complex128 line propagator is for 1-D plans");
   int rc = upload((void**)&p->prop128, prop, sizeof(double2) * p->d.ncomp * p->d.nx);
   if (rc) return rc;
@@ -589,14 +677,12 @@ int nlse_plan_set_propagator_c128(nlse_plan* p, const double* prop) {
 
 int nlse_plan_set_propagator_t(nlse_plan* p, const double* pt) {
   if (!p || !pt) return fail(NLSE_ERR_ARG, "null argument");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "temporal propagator factor is for 3-D plans");
   return upload((void**)&p->pt, pt, sizeof(double2) * p->d.nt);
 }
 
 int nlse_plan_set_propagator_3d_c128(nlse_plan* p, const void* tab) {
   if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 3) return fail(NLSE_ERR_ARG, "dense complex128 propagators are for 3-D plans");
   p->ptab128 = (const double2*)tab;
   p->pkind = PROP_C128;
@@ -605,7 +691,6 @@ int nlse_plan_set_propagator_3d_c128(nlse_plan* p, const void* tab) {
 
 int nlse_plan_set_propagator_c64(nlse_plan* p, const void* tab) {
   if (!p || !tab) return fail(NLSE_ERR_ARG, "null argument");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "dense complex64 propagators are for 2-D plans");
   p->ptab = (const float2*)tab;
   p->pkind = PROP_C64;
@@ -616,7 +701,6 @@ static float inv_or_zero(double v) { return std::isinf(v) ? 0.f : (float)(1.0 / 
 
 int nlse_plan_set_nl1(nlse_plan* p, int slot, const nlse_nl1* prm) {
   if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl1 arguments");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ncomp != 1) return fail(NLSE_ERR_ARG, "set_nl1 on a two-component plan");
   std::vector<NLParams1> h(p->d.items);
   for (int64_t i = 0; i < p->d.items; ++i) {
@@ -630,7 +714,6 @@ int nlse_plan_set_nl1(nlse_plan* p, int slot, const nlse_nl1* prm) {
 
 int nlse_plan_set_nl2(nlse_plan* p, int slot, const nlse_nl2* prm) {
   if (!p || !prm || slot < 0 || slot > 2) return fail(NLSE_ERR_ARG, "bad set_nl2 arguments");
-  graph_free(p);  // captured launches bake in table pointers
   if (p->d.ncomp != 2) return fail(NLSE_ERR_ARG, "set_nl2 on a one-component plan");
   std::vector<NLParams2> h(p->d.items * 2);
   for (int64_t i = 0; i < p->d.items * 2; ++i) {
@@ -646,7 +729,6 @@ int nlse_plan_set_nl2(nlse_plan* p, int slot, const nlse_nl2* prm) {
 
 int nlse_plan_set_rabi(nlse_plan* p, const double* cm1, const double* sn) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
-  graph_free(p);  // captured launches bake in table pointers
   if (!cm1 || !sn) {
     (void)hipFree(p->rabi);
     p->rabi = nullptr;
@@ -662,7 +744,6 @@ int nlse_plan_set_rabi(nlse_plan* p, const double* cm1, const double* sn) {
 
 int nlse_plan_set_potential(nlse_plan* p, const void* V, int vkind, int64_t vstride) {
   if (!p) return fail(NLSE_ERR_ARG, "null plan");
-  graph_free(p);  // captured launches bake in table pointers
   if (vkind != V_NONE && vkind != V_REAL && vkind != V_CPLX) return fail(NLSE_ERR_ARG, "bad v_kind %d", vkind);
   if (vkind != V_NONE && !V) return fail(NLSE_ERR_ARG, "null V");
   p->V = V;
@@ -1193,6 +1274,10 @@ int nlse_run(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t step0
 // first sweep, the last step and the remainder are launched directly.  The
 // key holds every pointer and flag the captured launches bake in, so a
 // change of tables re-captures.
+// Everything a captured step bakes into its launches: the field, every table
+// and parameter pointer and the kinds that select kernels.  The values behind
+// the pointers (propagator factors, nonlinear records, V) are read when the
+// graph replays, so setters that rewrite a table in place keep the graph.
 static std::vector<uint64_t> graph_key(const nlse_plan* p, const void* A, int precision, int64_t chunk) {
   std::vector<uint64_t> k = {(uint64_t)(uintptr_t)A, (uint64_t)precision, (uint64_t)chunk, (uint64_t)p->pkind,
                              (uint64_t)(uintptr_t)p->px, (uint64_t)(uintptr_t)p->py,

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _device as D
-from . import _lib
+from . import _lib, kernels_hip
 from .cnlse import CNLSE
 from .engine import NL2_DTYPE, SLOT_FULL, SLOT_HALF_POST, SLOT_HALF_PRE, inv_or_zero
 from .nlse import NLSE, bcast_items
@@ -62,6 +62,36 @@ class DDGPE(CNLSE):
         """Pump and probe drive of the cavity field (ddgpe.py:127-156)."""
         _drive(A, F_pump_r, F_pump_t[i], simu.delta_z)
         _drive(A, F_probe_r, F_probe_t[i], simu.delta_z)
+
+    @staticmethod
+    def add_noise(simu, A, t: float, i: int, noise: float = 0) -> None:
+        """ddgpe.py:96-125 on the device: A[..., 0] += noise sqrt(gamma/(4 dX dY))
+        rand1, A[..., 1] += noise sqrt(gamma2/(4 dX dY)) rand2 with rand =
+        N(0, delta_z) + 1j N(0, delta_z), one (NY, NX) draw shared by all
+        realisations.  The draw comes from Philox4x64-10 (nlse_add_noise):
+        key from ``simu.noise_seed`` (random when None), counter advanced on
+        every call, so a seeded run is reproducible; it cannot equal the
+        stream of cupy.random.normal."""
+        key = simu._noise_key()
+        counter = (0, simu._noise_calls, 0, 0)
+        simu._noise_calls += 1
+        cell = 4 * (simu.delta_X * simu.delta_Y)
+        gam = [np.asarray(D.to_host(x), dtype=np.float64) for x in (simu.gamma, simu.gamma2)]
+        if any(x.size != 1 for x in gam):
+            raise NotImplementedError("add_noise takes scalar gamma, gamma2")
+        amp0 = noise * np.sqrt(float(gam[0].reshape(-1)[0]) / cell)
+        amp1 = noise * np.sqrt(float(gam[1].reshape(-1)[0]) / cell)
+        kernels_hip.add_noise(A, simu.delta_z, amp0, amp1, counter, key)
+
+    noise_seed = None
+    _noise_calls = 0
+
+    def _noise_key(self) -> tuple:
+        if getattr(self, "_noise_key_words", None) is None:
+            seed = self.noise_seed if self.noise_seed is not None else np.random.SeedSequence().entropy
+            seed = int(seed) % (1 << 128)
+            self._noise_key_words = (seed & 0xFFFFFFFFFFFFFFFF, seed >> 64)
+        return self._noise_key_words
 
     _EXTRA = ("gamma", "g", "omega", "k_z", "omega_exc", "omega_cav", "detuning", "omega_pump")
 

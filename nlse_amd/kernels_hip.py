@@ -179,3 +179,90 @@ def sumsq(A: torch.Tensor, tdims: int) -> torch.Tensor:
     out = torch.empty(items, dtype=torch.float64, device=A.device)
     _lib.check(_lib.load().nlse_sumsq(A.data_ptr(), items, isz, out.data_ptr(), D.stream_ptr()), "sumsq")
     return out.reshape(A.shape[:A.dim() - tdims])
+
+
+# ------------------------------------------------ callbacks and helpers
+_NUM_KINDS = {torch.float32: 0, torch.float64: 1, torch.int32: 2, torch.int64: 3}
+
+
+def _delta_n_args(A, n2, I_sat, tdims: int, adapt: bool):
+    from scipy.constants import c, epsilon_0
+
+    _check_field(A)
+    items, isz = _items(A, tdims)
+    lead = tuple(A.shape[:A.dim() - tdims])
+    n2h = np.asarray(D.to_host(n2), dtype=np.float64)
+    coef = np.abs(n2h) if adapt else c * epsilon_0 / 2 * n2h
+    tc, pc = _param(coef, lead, items)
+    ti, pi = _param(I_sat, lead, items)
+    qc, qe = (c, epsilon_0) if adapt else (0.0, 0.0)
+    return items, isz, qc, qe, (tc, pc), (ti, pi)
+
+
+def delta_n(A: torch.Tensor, n2, I_sat, *, tdims: int = 2) -> torch.Tensor:
+    """c eps0/2 n2 |A|^2 / (1 + |A|^2/I_sat) in FP32 for every point
+    (callbacks.py:76-82), as a float32 device tensor of A's shape."""
+    items, isz, qc, qe, (tc, pc), (ti, pi) = _delta_n_args(A, n2, I_sat, tdims, adapt=False)
+    out = torch.empty(A.shape, dtype=torch.float32, device=A.device)
+    _lib.check(_lib.load().nlse_delta_n(A.data_ptr(), items, isz, qc, qe, pc, pi, out.data_ptr(), D.stream_ptr()),
+               "delta_n")
+    del tc, ti
+    return out
+
+
+def delta_n_max(A: torch.Tensor, n2, I_sat, *, tdims: int = 2) -> np.float32:
+    """max of |n2| q / (1 + q/I_sat) with q = re^2 + im^2 c eps0/2, the
+    operator precedence of callbacks.py:109, over all points (FP32, read
+    back to the host)."""
+    items, isz, qc, qe, (tc, pc), (ti, pi) = _delta_n_args(A, n2, I_sat, tdims, adapt=True)
+    out = torch.empty(1, dtype=torch.float32, device=A.device)
+    _lib.check(_lib.load().nlse_delta_n_max(A.data_ptr(), items, isz, qc, qe, pc, pi, out.data_ptr(),
+                                            D.stream_ptr()), "delta_n_max")
+    del tc, ti
+    return np.float32(out.item())
+
+
+def vortex_cp(im: torch.Tensor, i, j, ii: torch.Tensor, jj: torch.Tensor, ll: int) -> None:
+    """im += angle(((ii - i) + 1j*(jj - j))**ll) (kernels_gpu.py:159-176), in
+    place on a float32/float64 device image; ii, jj are the meshgrid
+    coordinates (float32/64 or int32/64), same number of elements."""
+    if not (isinstance(im, torch.Tensor) and im.is_cuda and im.is_contiguous() and im.dtype in (torch.float32,
+                                                                                                torch.float64)):
+        raise TypeError("im must be a contiguous float32/float64 device tensor")
+    ii = D.to_device(ii)
+    jj = D.to_device(jj)
+    if ii.dtype != jj.dtype or ii.dtype not in _NUM_KINDS or ii.numel() != im.numel() or jj.numel() != im.numel():
+        raise ValueError("ii and jj must be coordinate arrays of im's size and one numeric type")
+    _lib.check(_lib.load().nlse_vortex(im.data_ptr(), _NUM_KINDS[im.dtype], im.numel(), float(i), float(j),
+                                       ii.contiguous().data_ptr(), jj.contiguous().data_ptr(), _NUM_KINDS[ii.dtype],
+                                       int(ll), D.stream_ptr()), "vortex")
+
+
+def _u64(values, n: int):
+    import ctypes as C
+
+    arr = (C.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in values])
+    return arr
+
+
+def philox4x64(nblocks: int, counter=(0, 0, 0, 0), key=(0, 0)) -> torch.Tensor:
+    """4 * nblocks raw 64-bit words of Philox4x64-10 (the words of
+    numpy.random.Philox(counter=..., key=...).random_raw), as an int64
+    device tensor holding the uint64 bit patterns."""
+    out = torch.empty(4 * int(nblocks), dtype=torch.int64, device=D.device())
+    _lib.check(_lib.load().nlse_philox4x64(out.data_ptr(), int(nblocks), _u64(counter, 4), _u64(key, 2),
+                                           D.stream_ptr()), "philox4x64")
+    return out
+
+
+def add_noise(A: torch.Tensor, dz: float, amp0: float, amp1: float, counter, key) -> None:
+    """A[..., 0, :, :] += amp0 dz (z0 + 1j z1), A[..., 1, :, :] += amp1 dz (z2 + 1j z3)
+    with one (NY, NX) draw of standard normals shared by all realisations
+    (ddgpe.py:113-125); see nlse_add_noise."""
+    _check_field(A)
+    if A.dim() < 3 or A.shape[-3] != 2:
+        raise ValueError("add_noise needs a two-component field (..., 2, NY, NX)")
+    isz = A.shape[-1] * A.shape[-2]
+    items = A.numel() // (2 * isz)
+    _lib.check(_lib.load().nlse_add_noise(A.data_ptr(), items, isz, _u64(counter, 4), _u64(key, 2), float(dz),
+                                          float(amp0), float(amp1), D.stream_ptr()), "add_noise")

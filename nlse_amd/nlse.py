@@ -363,6 +363,12 @@ class NLSE:
 
     def _configure(self, eng: Plan, A: torch.Tensor, V, propagator, precision: str) -> None:
         self._configure_nonlocal(eng)
+        self._configure_step(eng, A, V, precision)
+        self._configure_tables(eng, V, propagator)
+
+    def _configure_step(self, eng: Plan, A: torch.Tensor, V, precision: str) -> None:
+        """The delta_z-dependent parameters: nonlinear records per slot and
+        the Rabi rotation (rewritten in place; captured steps see them)."""
         lead = self._param_lead(A)
         setter = eng.set_nl1 if self._ncomp() == 1 else eng.set_nl2
         for slot, rec in self._nl_slots(precision, lead, V).items():
@@ -370,7 +376,6 @@ class NLSE:
         if self._ncomp() == 2:
             r = self._rabi_params(lead)
             eng.set_rabi(*(r if r is not None else (None, None)))
-        self._configure_tables(eng, V, propagator)
 
     def _configure_nonlocal(self, eng: Plan) -> None:
         """nlse.py:406-409,444-447: with nl_length > 0 every nonlinear
@@ -466,8 +471,11 @@ class NLSE:
         t0 = time.perf_counter()
         if isinstance(self.delta_z, complex):
             raise NotImplementedError("imaginary-time evolution (complex delta_z) is not supported")
+        adaptive = self._adaptive_callback(callback, callback_args)
         callback, callback_args, fused = self._fuse_callbacks(callback, callback_args)
-        if callback is None:
+        if adaptive is not None:
+            self._loop_adaptive(A, V, z, precision, *adaptive)
+        elif callback is None:
             nsteps = count_steps(z, self.delta_z)
             if nsteps and z > self.L:
                 self.n2 = 0
@@ -487,6 +495,58 @@ class NLSE:
         if plot:
             self.plot_field(A, z)
         return A
+
+    def _adaptive_callback(self, callback, callback_args):
+        """(update_every, step-size list) when callbacks.adapt_delta_z is the
+        only callback, else None."""
+        from . import callbacks as CB
+
+        if callback is CB.adapt_delta_z:
+            args = callback_args
+        elif isinstance(callback, list) and len(callback) == 1 and callback[0] is CB.adapt_delta_z:
+            args = callback_args[0]
+        else:
+            return None
+        return tuple(args)
+
+    def _loop_adaptive(self, A, V, z: float, precision: str, update_every: int, dz_list: list) -> None:
+        """The reference loop (nlse.py:566-588) with adapt_delta_z
+        (callbacks.py:85-112) as its only callback, in chunks: the steps from
+        one update to the next share one delta_z and run as one engine call
+        (graph replay); each update reduces max delta_n on the device, reads
+        one float and rewrites the nonlinear records in place.  The step
+        sizes, their number and the z bookkeeping are those of the
+        step-by-step loop: after step i with i % update_every == 0 the new
+        delta_z is both recorded for later steps and added to z_prop."""
+        from . import callbacks as CB
+
+        if z > self.L:
+            self.n2 = 0
+        eng = self._engine_for(A)
+        self._configure(eng, A, V, self.propagator, precision)
+        flag = self._precision_flag(precision)
+        z_prop, i = 0.0, 0
+        while abs(z_prop) < z:
+            dz = self.delta_z
+            j = i if i % update_every == 0 else (i // update_every + 1) * update_every
+            count, zp = 0, z_prop
+            for s in range(i, j + 1):
+                if not abs(zp) < z:
+                    break
+                count += 1
+                if s == j:
+                    break
+                zp += dz
+            eng.run(A, count, flag, 0)
+            dz_list.extend([dz] * count)
+            i += count
+            if i - 1 == j:
+                CB.adapt_update(self, A)
+                zp += self.delta_z
+                self._configure_step(eng, A, V, precision)
+                if getattr(self, "adapt_rebuilds_propagator", False):
+                    self._configure_tables(eng, V, self.propagator)
+            z_prop = zp
 
     def _fuse_callbacks(self, callback, callback_args):
         """Callbacks that the engine runs inside the fused sweeps (none for

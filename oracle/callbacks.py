@@ -1,0 +1,92 @@
+"""Callbacks of the reference (NLSE/callbacks.py:57-112) and the device
+helpers of kernels_gpu.py:159-176 and ddgpe.py:96-125, restated in numpy.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Array arithmetic follows numpy 1.26 (the version the reference pins): a
+Python or numpy float scalar meets a float32 array as a float32 (value-based
+casting), so every elementwise operation below rounds to float32 in the
+order the reference's expression evaluates.
+
+Pinning (tests/test_oracle_pins.py):
+  * delta_n / adapt_delta_z: a constant field gives the closed form
+    |n2| q / (1 + q/I_sat) with q = re^2 + im^2 c eps0/2 (and re^2 + im^2
+    without the precedence quirk), and dz = 1/(12 k delta_n); the loop that
+    calls it is the reference loop itself (oracle/solvers.py out_field);
+  * vortex: charge 1 gives atan2(y, x); the winding number around the core
+    is the charge;
+  * philox_raw: bit-exact against numpy.random.Philox (numpy's own
+    Philox4x64-10); normals: Box-Muller identities and sample moments.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.constants import c, epsilon_0
+
+f32 = np.float32
+
+
+def delta_n(A: np.ndarray, n2: float, I_sat: float) -> np.ndarray:
+    """callbacks.py:80-82: c eps0/2 n2 A_sq / (1 + A_sq/I_sat), A_sq float32."""
+    re = A.real.astype(f32)
+    im = A.imag.astype(f32)
+    A_sq = re * re + im * im
+    coef = f32(c * epsilon_0 / 2 * n2)
+    return (coef * A_sq) / (f32(1) + A_sq / f32(I_sat))
+
+
+def delta_n_adapt(A: np.ndarray, n2: float, I_sat: float) -> np.ndarray:
+    """callbacks.py:109-110, with its precedence: only im^2 is scaled by
+    c eps0/2."""
+    re = A.real.astype(f32)
+    im = A.imag.astype(f32)
+    A_sq = re * re + ((im * im) * f32(c)) * f32(epsilon_0) / f32(2)
+    return (f32(abs(n2)) * A_sq) / (f32(1) + A_sq / f32(I_sat))
+
+
+def adapt_delta_z(simu, A, z: float, i: int, update_every: int, delta_z: list) -> None:
+    """callbacks.py:85-112 (the cached propagator is not rebuilt)."""
+    delta_z.append(simu.delta_z)
+    if i % update_every == 0:
+        m = delta_n_adapt(A, simu.n2, simu.I_sat).max()
+        with np.errstate(divide="ignore"):
+            z_nl = float(1 / (simu.k * np.float64(m)))
+        simu.delta_z = np.abs(z_nl) / 12
+
+
+def vortex(im: np.ndarray, i: float, j: float, ii: np.ndarray, jj: np.ndarray, ll: int) -> None:
+    """kernels_gpu.py:159-176: im += angle(((ii - i) + 1j*(jj - j))**ll)
+    (complex128 power, as cupy evaluates it)."""
+    z = (ii.astype(np.float64) - i) + 1j * (jj.astype(np.float64) - j)
+    im += np.angle(z**ll).astype(im.dtype)
+
+
+def philox_raw(nblocks: int, counter=(0, 0, 0, 0), key=(0, 0)) -> np.ndarray:
+    """4 * nblocks words of numpy.random.Philox (Philox4x64-10) for a
+    128-bit key and 256-bit counter given as 64-bit words."""
+    ctr = sum(int(v) << (64 * k) for k, v in enumerate(counter))
+    k = sum(int(v) << (64 * n) for n, v in enumerate(key))
+    return np.random.Philox(counter=ctr, key=k).random_raw(4 * nblocks).astype(np.uint64)
+
+
+def box_muller(w1: np.ndarray, w2: np.ndarray):
+    """Two standard normals from two 64-bit words (53-bit uniforms u;
+    sqrt(-2 ln(1 - u1)) times cos / sin(2 pi u2))."""
+    u1 = (w1 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u2 = (w2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    r = np.sqrt(-2.0 * np.log(1.0 - u1))
+    return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+
+
+def add_noise(A: np.ndarray, dz: float, amp0: float, amp1: float, counter, key) -> None:
+    """ddgpe.py:113-125 with the draws of nlse_add_noise: point p of the
+    (NY, NX) grid takes Philox block p; A (..., 2, NY, NX) complex64."""
+    ny, nx = A.shape[-2:]
+    w = philox_raw(ny * nx, counter, key).reshape(ny * nx, 4)
+    z0, z1 = box_muller(w[:, 0], w[:, 1])
+    z2, z3 = box_muller(w[:, 2], w[:, 3])
+    rand1 = (dz * z0 + 1j * (dz * z1)).reshape(ny, nx)
+    rand2 = (dz * z2 + 1j * (dz * z3)).reshape(ny, nx)
+    A[..., 0, :, :] += amp0 * rand1
+    A[..., 1, :, :] += amp1 * rand2

@@ -330,3 +330,80 @@ def test_lorentzian_response_closed_form():
     out = o._convolution(I)
     expect = 1 + np.cos(k0 * np.arange(64) * o.delta_X)[None, :] / (1 + k0**2 * 30e-6**2)
     assert np.abs(out - expect).max() < 1e-12
+
+
+def test_delta_n_closed_form_and_precedence():
+    """oracle.callbacks: a constant field gives |n2| q/(1 + q/I_sat) with
+    q = re^2 + im^2 c eps0/2 (adapt) or re^2 + im^2 (evaluate), and the
+    adaptive update dz = 1/(12 k delta_n); dropping the precedence quirk
+    changes the value by orders of magnitude (so the pin separates them)."""
+    import oracle.callbacks as CB
+
+    a, b, n2, isat = 3e3, 4e3, -1.6e-9, 1e5
+    A = np.full((8, 8), a + 1j * b, np.complex64)
+    q_adapt = a * a + b * b * c * epsilon_0 / 2
+    q_eval = a * a + b * b
+    want_adapt = abs(n2) * q_adapt / (1 + q_adapt / isat)
+    want_eval = c * epsilon_0 / 2 * n2 * q_eval / (1 + q_eval / isat)
+    assert np.allclose(CB.delta_n_adapt(A, n2, isat), want_adapt, rtol=2e-6)
+    assert np.allclose(CB.delta_n(A, n2, isat), want_eval, rtol=2e-6)
+    assert abs(want_adapt - abs(n2) * q_eval / (1 + q_eval / isat)) > 1e-3 * want_adapt
+    assert np.allclose(CB.delta_n_adapt(A, n2, np.inf), abs(n2) * q_adapt, rtol=2e-6)
+
+    class S:
+        pass
+
+    s = S()
+    s.n2, s.I_sat, s.k, s.delta_z = n2, isat, 2 * np.pi / 780e-9, 1e-5
+    hist = []
+    for i in range(3):
+        CB.adapt_delta_z(s, A, 1.0, i, 2, hist)
+    assert hist[0] == 1e-5 and hist[1] == hist[2]
+    assert np.isclose(s.delta_z, 1 / (s.k * want_adapt) / 12, rtol=2e-6)
+
+
+def test_vortex_closed_form_and_winding():
+    """oracle.callbacks.vortex: charge 1 is atan2(y, x); along a loop around
+    the core the phase winds by 2 pi * charge."""
+    import oracle.callbacks as CB
+
+    jj, ii = np.meshgrid(np.arange(64), np.arange(48))
+    im = np.zeros((48, 64))
+    CB.vortex(im, 20.0, 30.0, ii, jj, 1)
+    assert np.allclose(im, np.arctan2(jj - 30.0, ii - 20.0), atol=1e-12)
+    for ll in (-2, 3):
+        im = np.zeros((48, 64))
+        CB.vortex(im, 20.0, 30.0, ii, jj, ll)
+        loop = [(20 + 5, 30 + t) for t in range(-5, 5)] + [(20 - t, 35) for t in range(-5, 5)] + \
+               [(15, 30 - t) for t in range(-5, 5)] + [(20 + t, 25) for t in range(-5, 5)]
+        ph = np.array([im[p] for p in loop + loop[:1]])
+        d = np.angle(np.exp(1j * np.diff(ph)))
+        assert round(d.sum() / (2 * np.pi)) == ll
+
+
+def test_philox_and_box_muller():
+    """oracle.callbacks.philox_raw is numpy.random.Philox; an independent
+    big-integer restatement of Philox4x64-10 (Salmon et al. 2011) agrees on
+    counters with carries; Box-Muller normals have unit variance."""
+    import oracle.callbacks as CB
+
+    M = 2**64
+
+    def block(ctr, k, b):
+        v = (sum(x << (64 * i) for i, x in enumerate(ctr)) + b + 1) % 2**256
+        x = [(v >> (64 * i)) % M for i in range(4)]
+        k0, k1 = k
+        for r in range(10):
+            if r:
+                k0, k1 = (k0 + 0x9E3779B97F4A7C15) % M, (k1 + 0xBB67AE8584CAA73B) % M
+            p0, p1 = 0xD2E7470EE14C6C93 * x[0], 0xCA5A826395121157 * x[2]
+            x = [(p1 >> 64) ^ x[1] ^ k0, p1 % M, (p0 >> 64) ^ x[3] ^ k1, p0 % M]
+        return x
+
+    for ctr, key in [((0, 0, 0, 0), (0, 0)), ((M - 2, M - 1, 3, 0), (M - 1, 1))]:
+        raw = CB.philox_raw(3, ctr, key)
+        assert [int(w) for w in raw] == [w for b in range(3) for w in block(ctr, key, b)]
+    w = CB.philox_raw(50000, (1, 2, 0, 0), (7, 9)).reshape(-1, 4)
+    z0, z1 = CB.box_muller(w[:, 0], w[:, 1])
+    z = np.concatenate([z0, z1])
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.02 and abs(np.mean(z0 * z1)) < 0.01
