@@ -35,6 +35,10 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
+# extra compiler options (e.g. -DNLSE_P8_FROM_ROW=1024 to move the row
+# kernels' per-thread set size), appended after the defaults
+FLAGS += os.environ.get("NLSE_BUILD_FLAGS", "").split()
+
 # hipcc's host pass does not know the packed-fp32 target feature: harmless
 NOISE = ("packed-fp32-ops", "is not a recognized feature")
 

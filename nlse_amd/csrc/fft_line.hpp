@@ -181,6 +181,29 @@ struct Plan {
   static constexpr int TW_SIZE = tw_offset(NSTAGES);
 };
 
+// Twiddle records of the stages of Plan<n, p> (0 when p > n).
+__host__ __device__ constexpr int tw_size_np(int n, int p) {
+  if (p > n) return 0;
+  const int logn = ilog2c(n), logp = ilog2c(p);
+  const int rem = logn % logp, nst = logn / logp + (rem ? 1 : 0);
+  int off = 0, ns = rem ? (1 << rem) : p;
+  for (int i = 1; i < nst; ++i) {
+    off += (p - 1) * ns;
+    ns *= p;
+  }
+  return off;
+}
+
+// One twiddle table per line length serves every per-thread set size: the
+// stage tables of P = 16, 8 and 4 are stored one after the other, so
+// kernels of one length with different P share a table.
+__host__ __device__ constexpr int tw_base(int n, int p) {
+  return (p < 16 ? tw_size_np(n, 16) : 0) + (p < 8 ? tw_size_np(n, 8) : 0);
+}
+__host__ __device__ constexpr int tw_union_size(int n) {
+  return tw_size_np(n, 16) + tw_size_np(n, 8) + tw_size_np(n, 4);
+}
+
 // LDS line layout: one float2 of padding every 2^PADSH entries, plus EXTRA
 // float2 at the end of each line (staggers the lines of a multi-line
 // block across banks).
@@ -208,7 +231,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[G][P], int t, const float4
   static_assert(G % GR == 0, "lines per round must divide the lines per thread");
   float4 tws[NB][R > 1 ? R - 1 : 1];
   if constexpr (NS > 1) {
-    const float4* twst = tw + PL::tw_offset(S);
+    const float4* twst = tw + tw_base(N, P) + PL::tw_offset(S);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int jm = (t + b * T) & (NS - 1);

@@ -31,33 +31,44 @@ struct LineOps {
 };
 
 // ------------------------------------------------------ twiddle tables
-// One table per (device, line length), layout of Plan<N,P>::tw_offset:
-// stage s (s >= 1), entry [(k-1)*Ns + jm] = exp(-2 pi i jm k / (Ns R)),
-// evaluated in FP64 and stored as {c_hi, s_hi, c_lo, s_lo} with
-// c = c_hi + c_lo to ~2^-48.
+// One table per (device, line length): the stage tables of Plan<N, P> for
+// P = 16, 8, 4 in that order (tw_base), each in the layout of
+// Plan<N,P>::tw_offset: stage s (s >= 1), entry [(k-1)*Ns + jm] =
+// exp(-2 pi i jm k / (Ns R)), evaluated in FP64 and stored as
+// {c_hi, s_hi, c_lo, s_lo} with c = c_hi + c_lo to ~2^-48.
 template <int N, int P>
-std::vector<float4> build_twiddles() {
-  using PL = Plan<N, P>;
-  std::vector<float4> tw(PL::TW_SIZE > 0 ? PL::TW_SIZE : 1);
-  for (int s = 1; s < PL::NSTAGES; ++s) {
-    const int R = PL::radix(s), NS = PL::ns(s), off = PL::tw_offset(s);
-    for (int k = 1; k < R; ++k)
-      for (int jm = 0; jm < NS; ++jm) {
-        // angle = -2 pi * jm*k/(NS*R); reduce the rational first
-        const long num = (long)jm * k, den = (long)NS * R;
-        const double ang = -2.0 * M_PI * (double)num / (double)den;
-        const double c = std::cos(ang), sn = std::sin(ang);
-        const float ch = (float)c, sh = (float)sn;
-        tw[off + (k - 1) * NS + jm] = make_float4(ch, sh, (float)(c - (double)ch), (float)(sn - (double)sh));
-      }
+void build_stage_twiddles(std::vector<float4>& tw) {
+  if constexpr (P <= N) {
+    using PL = Plan<N, P>;
+    const int base = tw_base(N, P);
+    for (int s = 1; s < PL::NSTAGES; ++s) {
+      const int R = PL::radix(s), NS = PL::ns(s), off = base + PL::tw_offset(s);
+      for (int k = 1; k < R; ++k)
+        for (int jm = 0; jm < NS; ++jm) {
+          // angle = -2 pi * jm*k/(NS*R); reduce the rational first
+          const long num = (long)jm * k, den = (long)NS * R;
+          const double ang = -2.0 * M_PI * (double)num / (double)den;
+          const double c = std::cos(ang), sn = std::sin(ang);
+          const float ch = (float)c, sh = (float)sn;
+          tw[off + (k - 1) * NS + jm] = make_float4(ch, sh, (float)(c - (double)ch), (float)(sn - (double)sh));
+        }
+    }
   }
+}
+
+template <int N>
+std::vector<float4> build_twiddles() {
+  std::vector<float4> tw(tw_union_size(N) > 0 ? tw_union_size(N) : 1);
+  build_stage_twiddles<N, 16>(tw);
+  build_stage_twiddles<N, 8>(tw);
+  build_stage_twiddles<N, 4>(tw);
   return tw;
 }
 
 static std::mutex g_tw_mu;
 static std::map<std::pair<int, int>, float4*> g_tw;  // (device, N) -> table
 
-template <int N, int P>
+template <int N>
 static int get_twiddles(const float4** out) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -68,7 +79,7 @@ static int get_twiddles(const float4** out) {
     *out = it->second;
     return NLSE_OK;
   }
-  std::vector<float4> h = build_twiddles<N, P>();
+  std::vector<float4> h = build_twiddles<N>();
   float4* d = nullptr;
   HIP_TRY(hipMalloc(&d, h.size() * sizeof(float4)));
   HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -78,14 +89,84 @@ static int get_twiddles(const float4** out) {
 }
 
 // -------------------------------------------------- kernel dispatchers
+// Points per thread (P) of each kernel kind and line length: 16 by
+// default, 8 for lengths in [NLSE_P8_FROM_K, NLSE_P8_UPTO_K), 4 from
+// NLSE_P4_FROM_K.  A smaller P puts more waves on a line: the one-round
+// sweeps of a single field are latency-bound at two waves per SIMD, so most
+// row kernels of long lines run with P = 8.  Measured on MI355X (2048^2 and
+// 1024^2 steps): k_row and k_nlc_nl gain from P = 8 at 1024 and 2048,
+// k_nlc_z at 1024 only; the column kernel keeps P = 16.
+#ifndef NLSE_P8_FROM_COL
+#define NLSE_P8_FROM_COL (1 << 30)
+#endif
+#ifndef NLSE_P8_FROM_ROW
+#define NLSE_P8_FROM_ROW 1024
+#endif
+#ifndef NLSE_P8_FROM_NLZ
+#define NLSE_P8_FROM_NLZ 1024
+#endif
+#ifndef NLSE_P8_UPTO_NLZ
+#define NLSE_P8_UPTO_NLZ 2048
+#endif
+#ifndef NLSE_P8_FROM_NLNL
+#define NLSE_P8_FROM_NLNL 1024
+#endif
+#ifndef NLSE_P8_FROM_1D
+#define NLSE_P8_FROM_1D (1 << 30)
+#endif
+#ifndef NLSE_P8_UPTO_COL
+#define NLSE_P8_UPTO_COL (1 << 30)
+#endif
+#ifndef NLSE_P8_UPTO_ROW
+#define NLSE_P8_UPTO_ROW (1 << 30)
+#endif
+#ifndef NLSE_P8_UPTO_NLNL
+#define NLSE_P8_UPTO_NLNL (1 << 30)
+#endif
+#ifndef NLSE_P8_UPTO_1D
+#define NLSE_P8_UPTO_1D (1 << 30)
+#endif
+#ifndef NLSE_P4_FROM_COL
+#define NLSE_P4_FROM_COL (1 << 30)
+#endif
+#ifndef NLSE_P4_FROM_ROW
+#define NLSE_P4_FROM_ROW (1 << 30)
+#endif
+#ifndef NLSE_P4_FROM_NLZ
+#define NLSE_P4_FROM_NLZ (1 << 30)
+#endif
+#ifndef NLSE_P4_FROM_NLNL
+#define NLSE_P4_FROM_NLNL (1 << 30)
+#endif
+#ifndef NLSE_P4_FROM_1D
+#define NLSE_P4_FROM_1D (1 << 30)
+#endif
+
+constexpr int pick_p(int n, int p8_from, int p8_upto, int p4_from) {
+  return n >= p4_from && n >= 4 ? 4 : (n >= p8_from && n < p8_upto && n >= 8 ? 8 : (n >= 16 ? 16 : n));
+}
+#define NLSE_PICK_P(n, K) pick_p(n, NLSE_P8_FROM_##K, NLSE_P8_UPTO_##K, NLSE_P4_FROM_##K)
+constexpr int lines_per_block(int threads, int t) { return threads / t >= 1 ? threads / t : 1; }
+
 template <int N>
 struct LineCfg {
-  static constexpr int P = N >= 16 ? 16 : N;
+  // column kernel
+  static constexpr int P = NLSE_PICK_P(N, COL);
   static constexpr int T = N / P;
   static constexpr int LINE = LdsLine<N, 3, 3>::SIZE;  // column-pass lines
-  // rows per block: 256 threads for one component, 128 for pairs
-  static constexpr int LPB1 = (256 / T) >= 1 ? (256 / T) : 1;
-  static constexpr int LPB2 = (128 / T) >= 1 ? (128 / T) : 1;
+  // row kernel: 256 threads for one component, 128 for pairs
+  static constexpr int PR = NLSE_PICK_P(N, ROW);
+  static constexpr int LPB1 = lines_per_block(256, N / PR);
+  static constexpr int LPB2 = lines_per_block(128, N / PR);
+  // nonlocal row kernels (one line pair per N/P threads, 128 per block)
+  static constexpr int PZ = NLSE_PICK_P(N, NLZ);
+  static constexpr int LPBZ = lines_per_block(128, N / PZ);
+  static constexpr int PN = NLSE_PICK_P(N, NLNL);
+  static constexpr int LPBN = lines_per_block(128, N / PN);
+  // whole 1-D runs
+  static constexpr int P1 = NLSE_PICK_P(N, 1D);
+  static constexpr int LPB1D1 = lines_per_block(256, N / P1);
+  static constexpr int LPB1D2 = lines_per_block(128, N / P1);
   // default column tile: 8 columns (64-byte row segments; wider tiles
   // spill registers at 512 threads)
   static constexpr int WDEF = 8;
@@ -109,14 +190,15 @@ struct ColCfg {
 template <int N>
 static int row_launch(int G, const RowArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
+  constexpr int T = N / C::PR;
   if (G == 1) {
     constexpr int L = C::LPB1;
     const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_row<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+    hipLaunchKernelGGL((k_row<N, C::PR, L, 1>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
   } else {
     constexpr int L = C::LPB2;
     const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_row<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+    hipLaunchKernelGGL((k_row<N, C::PR, L, 2>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
   }
   return launch_status("k_row");
 }
@@ -124,36 +206,42 @@ static int row_launch(int G, const RowArgs& a, hipStream_t s) {
 template <int N>
 static int line1d_launch(int G, const Line1DArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
+  constexpr int T = N / C::P1;
   if (G == 1) {
-    constexpr int L = C::LPB1;
+    constexpr int L = C::LPB1D1;
     const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_line1d<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+    hipLaunchKernelGGL((k_line1d<N, C::P1, L, 1>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
   } else {
-    constexpr int L = C::LPB2;
+    constexpr int L = C::LPB1D2;
     const int64_t blocks = (a.ngroups + L - 1) / L;
-    hipLaunchKernelGGL((k_line1d<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+    hipLaunchKernelGGL((k_line1d<N, C::P1, L, 2>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
   }
   return launch_status("k_line1d");
 }
 
 // Nonlocal line kernels: stage 1 (k_nlc_z) or stage 2 (k_nlc_nl), one
-// line pair per T threads.
+// line pair per N/P threads.
 template <int N>
 static int nlc_launch(int stage, const NlcArgs& a, hipStream_t s) {
   using C = LineCfg<N>;
-  constexpr int L = C::LPB2;
-  const int64_t blocks = (a.ngroups + L - 1) / L;
   const bool full = a.full && a.ncomp == 1;
-  if (stage == 1 && full)
-    hipLaunchKernelGGL((k_nlc_z<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  else if (stage == 1)
-    hipLaunchKernelGGL((k_nlc_z<N, C::P, L>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  else if (full)
-    hipLaunchKernelGGL((k_nlc_nl<N, C::P, L, 1, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  else if (a.ncomp == 1)
-    hipLaunchKernelGGL((k_nlc_nl<N, C::P, L, 1>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_nlc_nl<N, C::P, L, 2>), dim3((unsigned)blocks), dim3(L * C::T), 0, s, a);
+  if (stage == 1) {
+    constexpr int L = C::LPBZ, T = N / C::PZ;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    if (full)
+      hipLaunchKernelGGL((k_nlc_z<N, C::PZ, L, 1>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_nlc_z<N, C::PZ, L>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
+  } else {
+    constexpr int L = C::LPBN, T = N / C::PN;
+    const int64_t blocks = (a.ngroups + L - 1) / L;
+    if (full)
+      hipLaunchKernelGGL((k_nlc_nl<N, C::PN, L, 1, 1>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
+    else if (a.ncomp == 1)
+      hipLaunchKernelGGL((k_nlc_nl<N, C::PN, L, 1>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_nlc_nl<N, C::PN, L, 2>), dim3((unsigned)blocks), dim3(L * T), 0, s, a);
+  }
   return launch_status(stage == 1 ? "k_nlc_z" : "k_nlc_nl");
 }
 
@@ -224,9 +312,9 @@ static int col_launch(int W, const ColArgs& a, hipStream_t s) {
 template <int N>
 LineOps make_ops() {
   using C = LineCfg<N>;
-  return LineOps{&get_twiddles<N, C::P>, &row_launch<N>,     &line1d_launch<N>, &nlc_launch<N>,
+  return LineOps{&get_twiddles<N>, &row_launch<N>,     &line1d_launch<N>, &nlc_launch<N>,
                  &col_launch<N>,         &col_width_for<N>, C::LPB1,           C::LPB2,
-                 C::T};
+                 N / C::PR};
 }
 
 

@@ -272,6 +272,27 @@ def test_nlse_nonlocal_2048_parity(gpu):
     assert err < PARITY
 
 
+@pytest.mark.parametrize("N,nl,precision", [(1024, 20e-6, "single"), (1024, 20e-6, "double"), (1024, 0, "double"),
+                                             (2048, 0, "single"), (2048, 20e-6, "double")])
+def test_nlse_long_line_parity(gpu, N, nl, precision):
+    """Line lengths whose row kernels run with 8 points per thread (1024 and
+    2048; the column kernel keeps 16), local and nonlocal, 10 steps against
+    the oracle."""
+    from nlse_amd import NLSE
+
+    kw = dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, V=None, L=10e-3, NX=N, NY=N, Isat=1e5,
+              nl_length=nl)
+    g, o = NLSE(**kw), oracle.NLSE(**kw)
+    for s in (g, o):
+        s.delta_z = 1e-5
+    G = np.exp(-(g.XX.astype(np.float64) ** 2 + g.YY.astype(np.float64) ** 2) / (waist / 3) ** 2)
+    E0 = ((G + 0.3) * _noise(G.shape, 21, amp=0.1)).astype(np.complex64)
+    out = g.out_field(E0, 1e-4, precision=precision, verbose=False)
+    err = rel_l2(out, o.out_field(E0, 1e-4, precision=precision))
+    print(f"NLSE {N}^2 nl_length={nl} {precision} 10 steps: {err:.2e}")
+    assert err < PARITY
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("ny,nx", [(128, 128), (64, 128)])
 def test_nlse_lorentzian_matches_oracle(gpu, precision, ny, nx):
