@@ -120,6 +120,7 @@ def kernel_bytes(P: int, N: int, r: int, local: bool, lorentzian: bool = False) 
         "band": (3, band * (8 + 4)),                      # field at band points, values out
         "colZ": (4, P * 8 + khat),                        # Z in/out, kernel spectrum
         "nlc_nl": (5, P * (8 + 4 + 4 + 8) + band * 4),    # field, Z, V in; field out
+        "band+colZ": (6, band * (8 + 4) + P * 8 + khat),  # the two above in one launch (as the step runs them)
     }
 
 

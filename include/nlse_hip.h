@@ -263,9 +263,10 @@ int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_
 int nlse_run_graph(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t chunk, void* stream);
 /* Launch one sweep type `count` times in place on A (device): pass 0 is a
  * steady-state row sweep (I_x, nonlinear step, F_x), pass 1 a column sweep
- * (F_y, propagator, I_y); with a nonlocal response, passes 2-5 are its
+ * (F_y, propagator, I_y); with a nonlocal response, passes 2-6 are its
  * launches: 2 I_x + intensity transform, 3 band convolution, 4 intensity
- * column sweep, 5 nonlinear step + F_x.  The data is meaningless
+ * column sweep, 5 nonlinear step + F_x, 6 passes 3 and 4 as a step runs
+ * them (one shared launch where the tile allows).  The data is meaningless
  * afterwards; this exists to time each kernel of the step in isolation. */
 int nlse_run_pass(nlse_plan* plan, void* A_dev, int pass, int64_t count, int precision, void* stream);
 /* Kernel configuration actually used (rows per block, columns per block). */

@@ -1046,11 +1046,26 @@ static NlcArgs plan_nlc_args(const nlse_plan* p, void* A) {
   return a;
 }
 
+static BandArgs band_args(const nlse_plan* p, const void* A);
+
 static int band_launch(const nlse_plan* p, const void* A, hipStream_t s) {
   const auto& n = p->nlc;
   const int64_t slabs = p->d.items * p->d.ncomp;
   if (n.ntiles == 0) return NLSE_OK;
   if (slabs > 65535) return fail(NLSE_ERR_ARG, "too many fields for the nonlocal band pass (%lld)", (long long)slabs);
+  const BandArgs b = band_args(p, A);
+  const dim3 grid((unsigned)n.ntiles, (unsigned)slabs), block(64 * BAND_WAVES);
+  if (n.pitch == 128)
+    hipLaunchKernelGGL(k_nl_band<128>, grid, block, n.lds, s, b);
+  else if (n.pitch == 256)
+    hipLaunchKernelGGL(k_nl_band<256>, grid, block, n.lds, s, b);
+  else
+    hipLaunchKernelGGL(k_nl_band<0>, grid, block, n.lds, s, b);
+  return launch_status("k_nl_band");
+}
+
+static BandArgs band_args(const nlse_plan* p, const void* A) {
+  const auto& n = p->nlc;
   BandArgs b;
   b.A = (const float2*)A;
   b.ny = (int)p->d.ny;
@@ -1060,18 +1075,12 @@ static int band_launch(const nlse_plan* p, const void* A, hipStream_t s) {
   b.kr = n.kr;
   b.kc = n.kc;
   b.tiles = n.tiles;
+  b.ntiles = n.ntiles;
   b.rowmap = n.rowmap;
   b.nb = n.nb;
   b.rowband = n.rowband;
   b.colband = n.colband;
-  const dim3 grid((unsigned)n.ntiles, (unsigned)slabs), block(64 * BAND_WAVES);
-  if (n.pitch == 128)
-    hipLaunchKernelGGL(k_nl_band<128>, grid, block, n.lds, s, b);
-  else if (n.pitch == 256)
-    hipLaunchKernelGGL(k_nl_band<256>, grid, block, n.lds, s, b);
-  else
-    hipLaunchKernelGGL(k_nl_band<0>, grid, block, n.lds, s, b);
-  return launch_status("k_nl_band");
+  return b;
 }
 
 static ColArgs nlc_col_args(const nlse_plan* p) {
@@ -1089,6 +1098,29 @@ static ColArgs nlc_col_args(const nlse_plan* p) {
 
 // One nonlinear application with the nonlocal response: [I_x,] the
 // convolved intensity, N, [Rabi, drive, F_x].  drive_step < 0: no drive.
+#ifndef NLSE_MERGE_BAND
+#define NLSE_MERGE_BAND 1
+#endif
+
+// Band pass and intensity column sweep.  Both only read what the first
+// nonlocal row pass wrote, so they share one launch where the column tile
+// allows it (k_colz_band: the VALU-bound band tiles run beside the
+// HBM-bound sweep); a second stream costs more in cross-queue waits than
+// it overlaps.
+static int band_and_z(nlse_plan* p, const void* A, hipStream_t s, const ColArgs* cz_override = nullptr) {
+  auto& n = p->nlc;
+  const int64_t slabs = p->d.items * p->d.ncomp;
+  const ColArgs cz = cz_override ? *cz_override : nlc_col_args(p);
+  int rc = NLSE_OK;
+  if (NLSE_MERGE_BAND && !n.direct && n.ntiles > 0 && slabs <= 65535) {
+    rc = n.zops->colz_band(n.colw, cz, band_args(p, A), n.pitch, n.lds, (int64_t)n.ntiles * slabs, s);
+    if (rc <= 0) return rc;  // launched, or an error
+  }
+  if ((rc = band_launch(p, A, s))) return rc;
+  if (!n.direct) rc = n.zops->col(n.colw, cz, s);
+  return rc;
+}
+
 static int nlc_apply(nlse_plan* p, void* A, int inverse_first, int forward_last, const void* nlp, bool rabi,
                      int64_t drive_step, hipStream_t s) {
   NlcArgs a = plan_nlc_args(p, A);
@@ -1097,10 +1129,7 @@ static int nlc_apply(nlse_plan* p, void* A, int inverse_first, int forward_last,
   auto& n = p->nlc;
   if (a.inverse_first || a.write_z) rc = p->rops->nlc(1, a, s);
   if (rc) return rc;
-  // one stream: running the band pass on a second stream beside the
-  // intensity sweep costs more in cross-queue waits than it overlaps
-  if ((rc = band_launch(p, A, s))) return rc;
-  if (!n.direct && (rc = n.zops->col(n.colw, nlc_col_args(p), s))) return rc;
+  if ((rc = band_and_z(p, A, s))) return rc;
   a.forward_last = forward_last;
   a.nlp = nlp;
   a.has_rabi = rabi ? 1 : 0;
@@ -1334,7 +1363,7 @@ int nlse_run_graph(nlse_plan* p, void* A, int64_t nsteps, int precision, int64_t
 }
 
 int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision, void* stream) {
-  if (!p || !A || count < 0 || pass < 0 || pass > 5 || (precision != 0 && precision != 1))
+  if (!p || !A || count < 0 || pass < 0 || pass > 6 || (precision != 0 && precision != 1))
     return fail(NLSE_ERR_ARG, "bad run_pass arguments");
   if (p->d.ndim != 2) return fail(NLSE_ERR_ARG, "run_pass times the sweeps of 2-D plans");
   int rc = check_ready(p, precision);
@@ -1355,6 +1384,7 @@ int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision,
       if (pass == 2) rc = p->rops->nlc(1, a, s);
       else if (pass == 3) rc = band_launch(p, A, s);
       else if (pass == 4) rc = p->nlc.zops->col(p->nlc.colw, cz, s);
+      else if (pass == 6) rc = band_and_z(p, A, s, &cz);
       else rc = p->rops->nlc(2, a, s);
     }
     return rc;

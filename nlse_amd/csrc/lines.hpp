@@ -13,6 +13,9 @@
 #include "common.hpp"
 #include "passes_2d.hpp"
 
+#ifndef NLSE_COL_THREADS
+#define NLSE_COL_THREADS 512  // threads of one column-sweep block at most
+#endif
 #ifndef NLSE_COL_LDS_KB
 #define NLSE_COL_LDS_KB 150  // LDS budget of one column-sweep block
 #endif
@@ -27,6 +30,9 @@ struct LineOps {
   int (*nlc)(int, const NlcArgs&, hipStream_t);
   int (*col)(int, const ColArgs&, hipStream_t);
   int (*colw)(int64_t, int);
+  // intensity column sweep + band pass in one launch (1: not available for
+  // this tile width, use the separate launches)
+  int (*colz_band)(int, const ColArgs&, const BandArgs&, int, size_t, int64_t, hipStream_t);
   int lpb1, lpb2, t;
 };
 
@@ -178,7 +184,7 @@ struct LineCfg {
 template <int N, int W>
 struct ColCfg {
   using C = LineCfg<N>;
-  static constexpr int WTMAX = (512 / C::T) >= 1 ? (512 / C::T) : 1;
+  static constexpr int WTMAX = (NLSE_COL_THREADS / C::T) >= 1 ? (NLSE_COL_THREADS / C::T) : 1;
   static constexpr int WT = W < WTMAX ? W : WTMAX;
   static constexpr int G = W / WT;
   static constexpr int GRMAX = (NLSE_COL_LDS_KB * 1024) / (WT * C::LINE * 8);
@@ -309,12 +315,71 @@ static int col_launch(int W, const ColArgs& a, hipStream_t s) {
   return launch_status("k_col");
 }
 
+template <int N, int W, int PITCH>
+static int colz_band_launch_wp(const ColArgs& c, const BandArgs& b, size_t band_lds, int64_t nband,
+                               hipStream_t s) {
+  using C = LineCfg<N>;
+  using K = ColCfg<N, W>;
+  constexpr int THREADS = K::WT * C::T;
+  if constexpr (K::OK && K::G == 1 && THREADS % (64 * BAND_WAVES) == 0) {
+    constexpr int TPB = THREADS / (64 * BAND_WAVES);
+    const int64_t ncol = c.nslabs * (c.NX / W);
+    const int64_t nbb = (nband + TPB - 1) / TPB;
+    const size_t col_bytes = sizeof(float2) * (size_t)col_lds_size<N, K::WT, 1>();
+    const size_t band_bytes = TPB * band_lds;
+    const size_t bytes = col_bytes > band_bytes ? col_bytes : band_bytes;
+    if (bytes > 160 * 1024) return 1;
+    auto fn = k_colz_band<N, C::P, K::WT, PITCH>;
+    if (bytes > 64 * 1024) {
+      // raise the dynamic LDS limit once per device
+      static bool done[64] = {};
+      int dev = 0;
+      HIP_TRY(hipGetDevice(&dev));
+      if (dev < 64 && !done[dev]) {
+        HIP_TRY(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        done[dev] = true;
+      }
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)(ncol + nbb)), dim3(THREADS), bytes, s, c, b, ncol, nband,
+                       (int)(band_lds / sizeof(float)));
+    return launch_status("k_colz_band");
+  } else {
+    return 1;
+  }
+}
+
+template <int N, int W>
+static int colz_band_launch_w(int pitch, const ColArgs& c, const BandArgs& b, size_t band_lds, int64_t nband,
+                              hipStream_t s) {
+  if (pitch == 128) return colz_band_launch_wp<N, W, 128>(c, b, band_lds, nband, s);
+  if (pitch == 256) return colz_band_launch_wp<N, W, 256>(c, b, band_lds, nband, s);
+  return colz_band_launch_wp<N, W, 0>(c, b, band_lds, nband, s);
+}
+
+// The shared launch uses 256-thread blocks (one band tile each, so the band
+// tiles keep the occupancy of their own kernel): column tiles of 256/T
+// columns, whatever the plan's column width.  Below 4 columns (32-byte row
+// segments, intensity columns of 2048 and more) the narrow tiles cost more
+// than the overlap gains (4096^2: 192 -> 210 us), and the launches stay
+// separate.
+template <int N>
+static int colz_band_launch(int, const ColArgs& c, const BandArgs& b, int pitch, size_t band_lds, int64_t nband,
+                            hipStream_t s) {
+  constexpr int W = 64 * BAND_WAVES / LineCfg<N>::T;
+  if constexpr (W >= 4 && W <= 64) {
+    if (c.NX % W != 0) return 1;
+    return colz_band_launch_w<N, W>(pitch, c, b, band_lds, nband, s);
+  } else {
+    return 1;
+  }
+}
+
 template <int N>
 LineOps make_ops() {
   using C = LineCfg<N>;
-  return LineOps{&get_twiddles<N>, &row_launch<N>,     &line1d_launch<N>, &nlc_launch<N>,
-                 &col_launch<N>,         &col_width_for<N>, C::LPB1,           C::LPB2,
-                 N / C::PR};
+  return LineOps{&get_twiddles<N>, &row_launch<N>,     &line1d_launch<N>,    &nlc_launch<N>,
+                 &col_launch<N>,    &col_width_for<N>, &colz_band_launch<N>, C::LPB1,
+                 C::LPB2,           N / C::PR};
 }
 
 

@@ -311,6 +311,34 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   col_body<N, P, WT, G, GR, PK>(a, b, lds);
 }
 
+// Intensity-spectrum column sweep (PROP_KREAL) and the band pass of the
+// nonlocal response in one launch: blocks [0, ncol) run column tiles of W =
+// WT columns, the others run blockDim/256 band tiles each, so the band
+// pass (VALU-bound) runs beside the sweep (HBM-bound) instead of after it.
+// The two read disjoint data (Z; the real-space field) and write disjoint
+// data (Z; the band values).  LDS is dynamic: the larger of the column
+// tile's and the band tiles' needs.
+template <int N, int P, int WT, int PITCH>
+__global__ __launch_bounds__(WT*(N / P)) void k_colz_band(ColArgs c, BandArgs b, int64_t ncol, int64_t nband,
+                                                          int band_floats) {
+  extern __shared__ float4 smem4[];
+  float* smem = reinterpret_cast<float*>(smem4);
+  if ((int64_t)blockIdx.x < ncol) {
+    // XCD-aware order as in k_col
+    const int64_t bx = blockIdx.x;
+    const int64_t tb = (ncol % 8 == 0) ? (bx % 8) * (ncol / 8) + bx / 8 : bx;
+    col_body<N, P, WT, 1, 1, PROP_KREAL>(c, tb, reinterpret_cast<float2*>(smem));
+  } else {
+    constexpr int TPB = WT * (N / P) / (64 * BAND_WAVES);
+    const int sub = threadIdx.x / (64 * BAND_WAVES);
+    const int64_t item = ((int64_t)blockIdx.x - ncol) * TPB + sub;
+    const bool real = item < nband;
+    const int64_t it = real ? item : nband - 1;
+    band_body<PITCH>(b, (int)(it % b.ntiles), it / b.ntiles, threadIdx.x % (64 * BAND_WAVES),
+                     smem + (size_t)sub * band_floats, real);
+  }
+}
+
 // ----------------------------------------------------------------- 1-D
 // Whole 1-D runs: every line is independent, so one launch performs all
 // `nsteps` steps of LPB lines (G = 1 or component pairs G = 2) in
