@@ -170,6 +170,12 @@ __device__ __forceinline__ float nlc_band(const NlcArgs& a, int bi, int64_t slab
 // one after the other (potential, nonlinear step, F_x, store), so only one
 // row's working set is live at a time; two components (NC = 2) couple at
 // every point and are transformed one line after the other.
+// Both rows of a one-component pair transformed by one two-line F_x
+// (measured 27.9 -> 27.0 us at 2048^2) rather than one after the other.
+#ifndef NLSE_NLNL_PAIR_FX
+#define NLSE_NLNL_PAIR_FX 1
+#endif
+
 template <int N, int P, int LPB, int NC, int FULL = 0>
 __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
   using PL = Plan<N, P>;
@@ -258,6 +264,31 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
     for (int m = 0; m < P; ++m) I[m] = z[0][m].x;
     load_row_v(L.y[0]);
     nl1_block_I<P>(v[0], I, p, a.loss_sat, vr, vi);
+#if NLSE_NLNL_PAIR_FX
+    // both rows' F_x in one two-line transform
+    float2 u[2][P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      u[0][m] = v[0][m];
+      u[1][m] = active ? a.A[L.off[1] + t + m * T] : make_float2(0.f, 0.f);
+      I[m] = z[0][m].y;
+    }
+    load_row_v(L.y[1]);
+    nl1_block_I<P>(u[1], I, p, a.loss_sat, vr, vi);
+    if (a.forward_last) fft_line<N, P, 2, false, 1>(u, t, a.tw, lline, LINE);
+    if (a.out_scale != 1.f) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int m = 0; m < P; ++m) u[g][m] = make_float2(u[g][m].x * a.out_scale, u[g][m].y * a.out_scale);
+    }
+    if (active) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int m = 0; m < P; ++m) a.A[L.off[g] + t + m * T] = u[g][m];
+    }
+#else
     finish_line(v, L.off[0]);
     float2 w[1][P];
 #pragma unroll
@@ -268,6 +299,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
     load_row_v(L.y[1]);
     nl1_block_I<P>(w[0], I, p, a.loss_sat, vr, vi);
     finish_line(w, L.off[1]);
+#endif
   } else {
     const NLParams2* pp = reinterpret_cast<const NLParams2*>(a.nlp) + 2 * L.item;
     const NLParams2 p1 = pp[0], p2 = pp[1];

@@ -21,6 +21,10 @@ namespace nlse {
 
 enum : int { PROP_SEP64 = 0, PROP_C64 = 1, PROP_SEP3D = 2, PROP_C128 = 3, PROP_KREAL = 4 };
 
+#ifndef NLSE_COL_PAIRED
+#define NLSE_COL_PAIRED 1
+#endif
+
 struct RowArgs {
   float2* A;
   int64_t ngroups;        // rows (G=1) or row pairs (G=2)
@@ -209,15 +213,26 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
   const int t = threadIdx.x / WT;
   const int tiles = a.NX / W;
   const int64_t slab = b / tiles;
-  const int c0 = (int)(b % tiles) * W + cw;
+  // column of line g: c0 + g * CS.  Two lines per thread take adjacent
+  // columns (CS = 1), so a thread moves 16 bytes per row and a wave's load
+  // covers whole 64-byte row segments; otherwise lines are WT apart.
+  constexpr int CS = (G == 2 && NLSE_COL_PAIRED) ? 1 : WT;
+  const int c0 = (int)(b % tiles) * W + (CS == 1 ? G * cw : cw);
   const int64_t base = slab * (int64_t)N * a.NX + c0;
   float2* lline = lds + cw * GR * LINE;
 
   float2 v[G][P];
 #pragma unroll
-  for (int m = 0; m < P; ++m)
+  for (int m = 0; m < P; ++m) {
+    if constexpr (CS == 1) {
+      const float4 q = *reinterpret_cast<const float4*>(a.A + base + (int64_t)(t + m * T) * a.NX);
+      v[0][m] = make_float2(q.x, q.y);
+      v[1][m] = make_float2(q.z, q.w);
+    } else {
 #pragma unroll
-    for (int g = 0; g < G; ++g) v[g][m] = a.A[base + (int64_t)(t + m * T) * a.NX + g * WT];
+      for (int g = 0; g < G; ++g) v[g][m] = a.A[base + (int64_t)(t + m * T) * a.NX + g * WT];
+    }
+  }
 
   if (a.do_fwd) fft_line<N, P, G, false, GR, 3>(v, t, a.tw, lline, LINE);
   const int pkind = PK >= 0 ? PK : a.pkind;
@@ -226,7 +241,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
     if ((PK < 0 || PK == PROP_SEP64) && pkind == PROP_SEP64) {
       double2 px[G];
 #pragma unroll
-      for (int g = 0; g < G; ++g) px[g] = a.px[comp * a.NX + c0 + g * WT];
+      for (int g = 0; g < G; ++g) px[g] = a.px[comp * a.NX + c0 + g * CS];
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const double2 py = a.py[comp * N + t + m * T];
@@ -239,7 +254,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
       double2 px[G], pt[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int col = c0 + g * WT;
+        const int col = c0 + g * CS;
         px[g] = a.px[col / a.nt];
         pt[g] = a.pt[col % a.nt];
       }
@@ -256,7 +271,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
       int kx[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int col = c0 + g * WT;
+        const int col = c0 + g * CS;
         kx[g] = col <= a.NX / 2 ? col : a.NX - col;
       }
 #pragma unroll
@@ -275,13 +290,13 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
       for (int m = 0; m < P; ++m)
 #pragma unroll
         for (int g = 0; g < G; ++g)
-          v[g][m] = prop_c128(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * WT], (double)a.scale);
+          v[g][m] = prop_c128(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * CS], (double)a.scale);
     } else if (PK < 0 || PK == PROP_C64) {
       const float2* pt = a.ptab + (int64_t)comp * N * a.NX + c0;
 #pragma unroll
       for (int m = 0; m < P; ++m)
 #pragma unroll
-        for (int g = 0; g < G; ++g) v[g][m] = prop_c64(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * WT], a.scale);
+        for (int g = 0; g < G; ++g) v[g][m] = prop_c64(v[g][m], pt[(int64_t)(t + m * T) * a.NX + g * CS], a.scale);
     }
   }
   if (a.do_inv) fft_line<N, P, G, true, GR, 3>(v, t, a.tw, lline, LINE);
@@ -293,9 +308,15 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
       for (int m = 0; m < P; ++m) v[g][m] = make_float2(v[g][m].x * a.out_scale, v[g][m].y * a.out_scale);
   }
 #pragma unroll
-  for (int m = 0; m < P; ++m)
+  for (int m = 0; m < P; ++m) {
+    if constexpr (CS == 1) {
+      *reinterpret_cast<float4*>(a.A + base + (int64_t)(t + m * T) * a.NX) =
+          make_float4(v[0][m].x, v[0][m].y, v[1][m].x, v[1][m].y);
+    } else {
 #pragma unroll
-    for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
+      for (int g = 0; g < G; ++g) a.A[base + (int64_t)(t + m * T) * a.NX + g * WT] = v[g][m];
+    }
+  }
 }
 
 // PK: the propagator kind compiled in (-1: any kind, chosen at run time).
