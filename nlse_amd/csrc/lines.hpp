@@ -315,60 +315,61 @@ static int col_launch(int W, const ColArgs& a, hipStream_t s) {
   return launch_status("k_col");
 }
 
-template <int N, int W, int PITCH>
+template <int N, int WT, int G, int PITCH>
 static int colz_band_launch_wp(const ColArgs& c, const BandArgs& b, size_t band_lds, int64_t nband,
                                hipStream_t s) {
   using C = LineCfg<N>;
-  using K = ColCfg<N, W>;
-  constexpr int THREADS = K::WT * C::T;
-  if constexpr (K::OK && K::G == 1 && THREADS % (64 * BAND_WAVES) == 0) {
-    constexpr int TPB = THREADS / (64 * BAND_WAVES);
-    const int64_t ncol = c.nslabs * (c.NX / W);
-    const int64_t nbb = (nband + TPB - 1) / TPB;
-    const size_t col_bytes = sizeof(float2) * (size_t)col_lds_size<N, K::WT, 1>();
-    const size_t band_bytes = TPB * band_lds;
-    const size_t bytes = col_bytes > band_bytes ? col_bytes : band_bytes;
-    if (bytes > 160 * 1024) return 1;
-    auto fn = k_colz_band<N, C::P, K::WT, PITCH>;
-    if (bytes > 64 * 1024) {
-      // raise the dynamic LDS limit once per device
-      static bool done[64] = {};
-      int dev = 0;
-      HIP_TRY(hipGetDevice(&dev));
-      if (dev < 64 && !done[dev]) {
-        HIP_TRY(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        done[dev] = true;
-      }
+  constexpr int W = WT * G;
+  constexpr int THREADS = WT * C::T;
+  static_assert(THREADS % (64 * BAND_WAVES) == 0, "shared launch blocks hold whole band tiles");
+  constexpr int TPB = THREADS / (64 * BAND_WAVES);
+  const int64_t ncol = c.nslabs * (c.NX / W);
+  const int64_t nbb = (nband + TPB - 1) / TPB;
+  const size_t col_bytes = sizeof(float2) * (size_t)col_lds_size<N, WT, 1>();
+  const size_t band_bytes = TPB * band_lds;
+  const size_t bytes = col_bytes > band_bytes ? col_bytes : band_bytes;
+  if (bytes > 160 * 1024) return 1;
+  auto fn = k_colz_band<N, C::P, WT, G, PITCH>;
+  if (bytes > 64 * 1024) {
+    // raise the dynamic LDS limit once per device
+    static bool done[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 64 && !done[dev]) {
+      HIP_TRY(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      done[dev] = true;
     }
-    hipLaunchKernelGGL(fn, dim3((unsigned)(ncol + nbb)), dim3(THREADS), bytes, s, c, b, ncol, nband,
-                       (int)(band_lds / sizeof(float)));
-    return launch_status("k_colz_band");
-  } else {
-    return 1;
   }
+  hipLaunchKernelGGL(fn, dim3((unsigned)(ncol + nbb)), dim3(THREADS), bytes, s, c, b, ncol, nband,
+                     (int)(band_lds / sizeof(float)));
+  return launch_status("k_colz_band");
 }
 
-template <int N, int W>
-static int colz_band_launch_w(int pitch, const ColArgs& c, const BandArgs& b, size_t band_lds, int64_t nband,
+template <int N, int WT, int G>
+static int colz_band_launch_g(int pitch, const ColArgs& c, const BandArgs& b, size_t band_lds, int64_t nband,
                               hipStream_t s) {
-  if (pitch == 128) return colz_band_launch_wp<N, W, 128>(c, b, band_lds, nband, s);
-  if (pitch == 256) return colz_band_launch_wp<N, W, 256>(c, b, band_lds, nband, s);
-  return colz_band_launch_wp<N, W, 0>(c, b, band_lds, nband, s);
+  if (pitch == 128) return colz_band_launch_wp<N, WT, G, 128>(c, b, band_lds, nband, s);
+  if (pitch == 256) return colz_band_launch_wp<N, WT, G, 256>(c, b, band_lds, nband, s);
+  return colz_band_launch_wp<N, WT, G, 0>(c, b, band_lds, nband, s);
 }
 
 // The shared launch uses 256-thread blocks (one band tile each, so the band
-// tiles keep the occupancy of their own kernel): column tiles of 256/T
-// columns, whatever the plan's column width.  Below 4 columns (32-byte row
-// segments, intensity columns of 2048 and more) the narrow tiles cost more
-// than the overlap gains (4096^2: 192 -> 210 us), and the launches stay
-// separate.
+// tiles keep the occupancy of their own kernel): column tiles of WT = 256/T
+// columns, one per thread (two adjacent columns per thread, which needs two
+// LDS rounds per stage within the band tiles' LDS, measured slower: 25.5 ->
+// 28.3 us at 2048^2).  Below 4 columns per tile (intensity columns of 2048
+// and more) the narrow tiles cost more than the overlap gains (4096^2:
+// 192 -> 210 us), and the launches stay separate.
+#ifndef NLSE_COLZ_G
+#define NLSE_COLZ_G 1
+#endif
 template <int N>
 static int colz_band_launch(int, const ColArgs& c, const BandArgs& b, int pitch, size_t band_lds, int64_t nband,
                             hipStream_t s) {
-  constexpr int W = 64 * BAND_WAVES / LineCfg<N>::T;
-  if constexpr (W >= 4 && W <= 64) {
-    if (c.NX % W != 0) return 1;
-    return colz_band_launch_w<N, W>(pitch, c, b, band_lds, nband, s);
+  constexpr int WT = 64 * BAND_WAVES / LineCfg<N>::T;
+  if constexpr (WT >= 2 && WT <= 64 && WT * NLSE_COLZ_G >= 4) {
+    if (c.NX % (WT * NLSE_COLZ_G) != 0) return 1;
+    return colz_band_launch_g<N, WT, NLSE_COLZ_G>(pitch, c, b, band_lds, nband, s);
   } else {
     return 1;
   }

@@ -334,12 +334,13 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 
 // Intensity-spectrum column sweep (PROP_KREAL) and the band pass of the
 // nonlocal response in one launch: blocks [0, ncol) run column tiles of W =
-// WT columns, the others run blockDim/256 band tiles each, so the band
+// WT*G columns (one LDS round per line), the others run blockDim/256 band
+// tiles each, so the band
 // pass (VALU-bound) runs beside the sweep (HBM-bound) instead of after it.
 // The two read disjoint data (Z; the real-space field) and write disjoint
 // data (Z; the band values).  LDS is dynamic: the larger of the column
 // tile's and the band tiles' needs.
-template <int N, int P, int WT, int PITCH>
+template <int N, int P, int WT, int G, int PITCH>
 __global__ __launch_bounds__(WT*(N / P)) void k_colz_band(ColArgs c, BandArgs b, int64_t ncol, int64_t nband,
                                                           int band_floats) {
   extern __shared__ float4 smem4[];
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(WT*(N / P)) void k_colz_band(ColArgs c, BandArgs b,
     // XCD-aware order as in k_col
     const int64_t bx = blockIdx.x;
     const int64_t tb = (ncol % 8 == 0) ? (bx % 8) * (ncol / 8) + bx / 8 : bx;
-    col_body<N, P, WT, 1, 1, PROP_KREAL>(c, tb, reinterpret_cast<float2*>(smem));
+    col_body<N, P, WT, G, 1, PROP_KREAL>(c, tb, reinterpret_cast<float2*>(smem));
   } else {
     constexpr int TPB = WT * (N / P) / (64 * BAND_WAVES);
     const int sub = threadIdx.x / (64 * BAND_WAVES);
