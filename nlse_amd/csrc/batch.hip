@@ -5,7 +5,7 @@
 //
 // Work units are uniform in size whatever the grid: a row unit is
 // 4096/NX rows of NX points (256 threads, 16 points each) and a column
-// unit is 8192/NY columns of NY points (512 threads).  Every unit therefore
+// unit is 8192/NY columns of NY points (256 threads, two columns each).  Every unit therefore
 // moves the same bytes and does the same FFT work per point, and a
 // persistent grid that strides over the unit list is balanced without a
 // work queue.  Each block reads its unit's instance descriptor (field
@@ -43,20 +43,35 @@ struct BatchFlags {
 };
 
 constexpr int BROW_THREADS = 256;
-constexpr int BCOL_THREADS = 512;
+// A column unit of 8192/NY columns is 512 lines of work: 512 threads with
+// one column each, or (NLSE_BATCH_COL_PAIRED) 256 threads owning two
+// adjacent columns each, moved with 16-byte accesses.
+#ifndef NLSE_BATCH_COL_PAIRED
+#define NLSE_BATCH_COL_PAIRED 1
+#endif
+constexpr int BCOL_UNIT = 512;
+constexpr int BCOL_G = NLSE_BATCH_COL_PAIRED ? 2 : 1;
+constexpr int BCOL_THREADS = BCOL_UNIT / BCOL_G;
 
+// Row units of 4096/NX rows run as BROW_PASSES row_body calls of LPB rows
+// with P points per thread.
+#ifndef NLSE_BATCH_ROW_P
+#define NLSE_BATCH_ROW_P 16
+#endif
 template <int N>
 struct BRow {
-  static constexpr int T = N / 16;
+  static constexpr int P = NLSE_BATCH_ROW_P;
+  static constexpr int T = N / P;
   static constexpr int LPB = BROW_THREADS / T;
+  static constexpr int PASSES = (4096 / N) / LPB;
   static constexpr int LDS = row_lds_size<N, LPB, 1>();
 };
 
 template <int N>
 struct BCol {
   static constexpr int T = N / 16;
-  static constexpr int WT = BCOL_THREADS / T;
-  static constexpr int LDS = col_lds_size<N, WT, 1>();
+  static constexpr int WT = BCOL_THREADS / T;  // threads across; W = WT * BCOL_G columns
+  static constexpr int LDS = col_lds_size<N, WT, BCOL_G>();
 };
 
 static bool batch_len_ok(int64_t n) { return n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048; }
@@ -82,8 +97,11 @@ __global__ __launch_bounds__(BROW_THREADS) void k_batch_row(const BatchInst* __r
     a.nlp_a = I.nl[f.slot_a];
     a.nlp_b = I.nl[f.slot_b];
     a.out_scale = f.timing ? I.inv_nx : 1.f;
-    row_body<N, 16, BRow<N>::LPB, 1>(a, un.first, lds);
-    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BRow<N>::PASSES; ++k) {
+      row_body<N, BRow<N>::P, BRow<N>::LPB, 1>(a, un.first + k * BRow<N>::LPB, lds);
+      __syncthreads();
+    }
   }
 }
 
@@ -101,7 +119,7 @@ __global__ __launch_bounds__(BCOL_THREADS) void k_batch_col(const BatchInst* __r
     const BatchInst& I = inst[un.inst];
     ColArgs c = I.col;
     if (f.timing) c.scale = I.inv_ny;
-    col_body<N, 16, BCol<N>::WT, 1, 1, PROP_SEP64>(c, un.first, lds);
+    col_body<N, 16, BCol<N>::WT, BCOL_G, BCOL_G, PROP_SEP64>(c, un.first, lds);
     __syncthreads();
   }
 }
