@@ -204,6 +204,31 @@ __host__ __device__ constexpr int tw_union_size(int n) {
   return tw_size_np(n, 16) + tw_size_np(n, 8) + tw_size_np(n, 4);
 }
 
+// The natural sets of G lines at element offsets off[g] (zeros for an
+// inactive group): one branch for the whole set, so the loads issue back to
+// back.
+template <int G, int P, int T>
+__device__ __forceinline__ void load_lines(const float2* __restrict__ A, const int64_t* off, int t, bool active,
+                                           float2 (&v)[G][P]) {
+  if (active) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = A[off[g] + t + m * T];
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[g][m] = make_float2(0.f, 0.f);
+  }
+}
+
+// Padding of the row kernels' LDS lines (one float2 every 2^NLSE_ROW_PADSH
+// entries); the column kernel passes its own.
+#ifndef NLSE_ROW_PADSH
+#define NLSE_ROW_PADSH 4
+#endif
+
 // LDS line layout: one float2 of padding every 2^PADSH entries, plus EXTRA
 // float2 at the end of each line (staggers the lines of a multi-line
 // block across banks).
@@ -289,7 +314,7 @@ __device__ __forceinline__ void fft_stages_from(float2 (&v)[G][P], int t, const 
 // Forward (INV=false) or unnormalised inverse (INV=true) DFT of G lines,
 // natural set in, natural set out.  The inverse is swap(F(swap(x))) with
 // swap(a + ib) = b + ia (= conj(F(conj x))), a register renaming.
-template <int N, int P, int G, bool INV, int GR = G, int PADSH = 4>
+template <int N, int P, int G, bool INV, int GR = G, int PADSH = NLSE_ROW_PADSH>
 __device__ __forceinline__ void fft_line(float2 (&v)[G][P], int t, const float4* __restrict__ tw,
                                          float2* lds, int lds_stride) {
   if constexpr (INV) {

@@ -100,7 +100,7 @@ template <int N, int P, int LPB, int FULL = 0>
 __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_z(NlcArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  constexpr int LINE = LdsLine<N, NLSE_ROW_PADSH, 0>::SIZE;
   __shared__ float2 lds[LPB * LINE];
   const int li = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -110,10 +110,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_z(NlcArgs a) {
   float2* lline = lds + li * LINE;
 
   float2 v[2][P];
-#pragma unroll
-  for (int g = 0; g < 2; ++g)
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[L.off[g] + t + m * T] : make_float2(0.f, 0.f);
+  load_lines<2, P, T>(a.A, L.off, t, active, v);
   if (a.inverse_first) {
     fft_line<N, P, 2, true, 1>(v, t, a.tw, lline, LINE);
     if (a.out_scale != 1.f) {
@@ -180,7 +177,7 @@ template <int N, int P, int LPB, int NC, int FULL = 0>
 __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  constexpr int LINE = LdsLine<N, NLSE_ROW_PADSH, 0>::SIZE;
   __shared__ float2 lds[LPB * LINE];
   const int li = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -199,10 +196,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
     for (int m = 0; m < P; ++m) z[0][m] = active ? a.Z[zrow * N + t + m * T] : make_float2(0.f, 0.f);
   }
   float2 v[NC][P];
-#pragma unroll
-  for (int g = 0; g < NC; ++g)
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[L.off[g] + t + m * T] : make_float2(0.f, 0.f);
+  load_lines<NC, P, T>(a.A, L.off, t, active, v);
   if (!a.direct) fft_line<N, P, 1, true>(z, t, a.tw, lline, LINE);
   // z.x: convolved intensity of line 0, z.y: of line 1
   if (active) {
@@ -223,16 +217,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
 
   float vr[P], vi[P];
   auto load_row_v = [&](int64_t y) {
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-      vr[m] = 0.f;
-      vi[m] = 0.f;
-    }
-    if (a.vkind != V_NONE) {
-      const int64_t vbase = L.item * a.v_item_stride + y * N + t;
-#pragma unroll
-      for (int m = 0; m < P; ++m) load_v(a.V, a.vkind, vbase + m * T, vr[m], vi[m]);
-    }
+    load_v_set<P, T>(a.V, a.vkind, L.item * a.v_item_stride + y * N + t, vr, vi);
   };
   auto finish_line = [&](float2 (&w)[1][P], int64_t off) {
     if (a.forward_last) fft_line<N, P, 1, false>(w, t, a.tw, lline, LINE);

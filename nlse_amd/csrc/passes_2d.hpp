@@ -55,7 +55,7 @@ struct RowArgs {
 // LDS floats2 of one row block.
 template <int N, int LPB, int G>
 constexpr int row_lds_size() {
-  return LPB * G * LdsLine<N, 4, 0>::SIZE;
+  return LPB * G * LdsLine<N, NLSE_ROW_PADSH, 0>::SIZE;
 }
 
 // Row sweep body: LPB line groups starting at group `g0`.
@@ -63,7 +63,7 @@ template <int N, int P, int LPB, int G>
 __device__ __forceinline__ void row_body(const RowArgs& a, int64_t g0, float2* lds) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  constexpr int LINE = LdsLine<N, NLSE_ROW_PADSH, 0>::SIZE;
 
   const int li = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -82,25 +82,13 @@ __device__ __forceinline__ void row_body(const RowArgs& a, int64_t g0, float2* l
   float2* lline = lds + li * G * LINE;
 
   float2 v[G][P];
-#pragma unroll
-  for (int g = 0; g < G; ++g)
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[off[g] + t + m * T] : make_float2(0.f, 0.f);
+  load_lines<G, P, T>(a.A, off, t, active, v);
 
   // side inputs of the real-space step, fetched with the field so that
   // their latency overlaps the inverse transform
   const bool has_nl = a.has_a || a.has_b;
   float vr[P], vi[P];
-#pragma unroll
-  for (int m = 0; m < P; ++m) {
-    vr[m] = 0.f;
-    vi[m] = 0.f;
-  }
-  if (has_nl && a.vkind != V_NONE) {
-    const int64_t vbase = item * a.v_item_stride + y * N + t;
-#pragma unroll
-    for (int m = 0; m < P; ++m) load_v(a.V, a.vkind, vbase + m * T, vr[m], vi[m]);
-  }
+  load_v_set<P, T>(a.V, has_nl ? a.vkind : V_NONE, item * a.v_item_stride + y * N + t, vr, vi);
 
   if (a.inverse_first) fft_line<N, P, G, true>(v, t, a.tw, lline, LINE);
 
@@ -389,7 +377,7 @@ template <int N, int P, int LPB, int G>
 __global__ __launch_bounds__(LPB*(N / P)) void k_line1d(Line1DArgs a) {
   using PL = Plan<N, P>;
   constexpr int T = PL::T;
-  constexpr int LINE = LdsLine<N, 4, 0>::SIZE;
+  constexpr int LINE = LdsLine<N, NLSE_ROW_PADSH, 0>::SIZE;
   __shared__ float2 lds[LPB * G * LINE];
   const int li = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -407,12 +395,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_line1d(Line1DArgs a) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[g][m] = active ? a.A[off[g] + t + m * T] : make_float2(0.f, 0.f);
   float vr[P], vi[P];
-#pragma unroll
-  for (int m = 0; m < P; ++m) {
-    vr[m] = 0.f;
-    vi[m] = 0.f;
-    if (a.vkind != V_NONE) load_v(a.V, a.vkind, t + m * T, vr[m], vi[m]);
-  }
+  load_v_set<P, T>(a.V, a.vkind, t, vr, vi);
   double2 pr[G][P];
 #pragma unroll
   for (int g = 0; g < G; ++g)

@@ -217,6 +217,35 @@ __device__ __forceinline__ void load_v(const void* V, int vkind, int64_t idx, fl
   }
 }
 
+// V at the P points base + m*T of a line (vr, vi zero without V).  The kind
+// is tested once for the whole set: with a per-point test the compiler
+// waits for each load at the branch join, serialising P load latencies.
+template <int P, int T>
+__device__ __forceinline__ void load_v_set(const void* V, int vkind, int64_t base, float* vr, float* vi) {
+  if (vkind == V_REAL) {
+    const float* p = reinterpret_cast<const float*>(V) + base;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      vr[m] = p[m * T];
+      vi[m] = 0.f;
+    }
+  } else if (vkind == V_CPLX) {
+    const float2* p = reinterpret_cast<const float2*>(V) + base;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const float2 w = p[m * T];
+      vr[m] = w.x;
+      vi[m] = w.y;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      vr[m] = 0.f;
+      vi[m] = 0.f;
+    }
+  }
+}
+
 // nl_prop: A *= exp(dz*(-alpha' [*sat] + i g I sat + i V'))
 // A complex V' = vr + i vi contributes i*V' = -vi + i vr.
 __device__ __forceinline__ float2 nl1(float2 A, const NLParams1& p, bool loss_sat, float vr, float vi) {
