@@ -349,8 +349,11 @@ static int colz_band_launch_wp(const ColArgs& c, const BandArgs& b, size_t band_
       done[dev] = true;
     }
   }
+  // band tiles first for long intensity columns (2048^2 fields: 25.6 ->
+  // 24.3 us), sweep tiles first below (1024^2: 8.9 vs 9.3 us)
+  const int band_first = N >= 1024 ? 1 : 0;
   hipLaunchKernelGGL(fn, dim3((unsigned)(ncol + nbb)), dim3(THREADS), bytes, s, c, b, ncol, nband,
-                     (int)(band_lds / sizeof(float)));
+                     (int)(band_lds / sizeof(float)), band_first);
   return launch_status("k_colz_band");
 }
 

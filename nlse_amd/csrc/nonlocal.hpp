@@ -331,7 +331,10 @@ struct BandTile {
   int y0, x0, nrows, ncols;
 };
 
-constexpr int RB_BAND = 4;     // output rows per lane
+#ifndef NLSE_RB_BAND
+#define NLSE_RB_BAND 4
+#endif
+constexpr int RB_BAND = NLSE_RB_BAND;  // output rows per lane
 constexpr int BAND_WAVES = 4;  // waves per block, splitting the kernel columns
 constexpr int KU_BAND = 4;     // kernel rows per register window
 

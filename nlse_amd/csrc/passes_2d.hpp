@@ -330,18 +330,22 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
 // tile's and the band tiles' needs.
 template <int N, int P, int WT, int G, int PITCH>
 __global__ __launch_bounds__(WT*(N / P)) void k_colz_band(ColArgs c, BandArgs b, int64_t ncol, int64_t nband,
-                                                          int band_floats) {
+                                                          int band_floats, int band_first) {
   extern __shared__ float4 smem4[];
   float* smem = reinterpret_cast<float*>(smem4);
-  if ((int64_t)blockIdx.x < ncol) {
+  constexpr int TPB = WT * (N / P) / (64 * BAND_WAVES);
+  const int64_t nbb = (nband + TPB - 1) / TPB;
+  // band_first: the long VALU-bound band tiles are dispatched before the
+  // sweep's tiles, which then fill in beside them
+  const int64_t bx = band_first ? ((int64_t)blockIdx.x < nbb ? (int64_t)blockIdx.x + ncol : (int64_t)blockIdx.x - nbb)
+                                : (int64_t)blockIdx.x;
+  if (bx < ncol) {
     // XCD-aware order as in k_col
-    const int64_t bx = blockIdx.x;
     const int64_t tb = (ncol % 8 == 0) ? (bx % 8) * (ncol / 8) + bx / 8 : bx;
     col_body<N, P, WT, G, 1, PROP_KREAL>(c, tb, reinterpret_cast<float2*>(smem));
   } else {
-    constexpr int TPB = WT * (N / P) / (64 * BAND_WAVES);
     const int sub = threadIdx.x / (64 * BAND_WAVES);
-    const int64_t item = ((int64_t)blockIdx.x - ncol) * TPB + sub;
+    const int64_t item = (bx - ncol) * TPB + sub;
     const bool real = item < nband;
     const int64_t it = real ? item : nband - 1;
     band_body<PITCH>(b, (int)(it % b.ntiles), it / b.ntiles, threadIdx.x % (64 * BAND_WAVES),
