@@ -178,7 +178,9 @@ def bench_c1(args) -> int:
     P = A.numel()
     r = 0 if args.local else int(tuple(sim.nl_profile.shape)[0] // 2)
     kb = kernel_bytes(P, N, r, args.local, lorentzian)
-    step_bytes = sum(b for _, b in kb.values())
+    # one step's launches: the shared band+colZ entry repeats the bytes of
+    # the band and colZ entries (timed alone for reference)
+    step_bytes = sum(b for name, (_, b) in kb.items() if name != "band+colZ")
 
     # warm-up, then K steps bracketed by synchronisation
     eng.run(A, args.warmup)
