@@ -266,15 +266,22 @@ static bool col_launch_w(const ColArgs& a, hipStream_t s) {
   using K = ColCfg<N, W>;
   if constexpr (K::OK) {
     const int64_t blocks = a.nslabs * (a.NX / W);
-    // the separable 2-D propagator and the real kernel spectrum of the
-    // nonlocal response get their own instances (fewer live registers
-    // than the run-time dispatch over every kind)
+    // the separable 2-D and 3-D propagators, the real kernel spectrum of
+    // the nonlocal response and the propagator-free transforms get their
+    // own instances (fewer live registers than the run-time dispatch over
+    // every kind)
     const bool prop = a.do_prop != 0;
     if (prop && a.pkind == PROP_SEP64)
       hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PROP_SEP64>), dim3((unsigned)blocks),
                          dim3(K::WT * C::T), 0, s, a);
     else if (prop && a.pkind == PROP_KREAL)
       hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PROP_KREAL>), dim3((unsigned)blocks),
+                         dim3(K::WT * C::T), 0, s, a);
+    else if (prop && a.pkind == PROP_SEP3D)
+      hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PROP_SEP3D>), dim3((unsigned)blocks),
+                         dim3(K::WT * C::T), 0, s, a);
+    else if (!prop)
+      hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR, PK_NONE>), dim3((unsigned)blocks),
                          dim3(K::WT * C::T), 0, s, a);
     else
       hipLaunchKernelGGL((k_col<N, C::P, K::WT, K::G, K::GR>), dim3((unsigned)blocks), dim3(K::WT * C::T), 0, s,

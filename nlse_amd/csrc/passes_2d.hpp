@@ -20,6 +20,8 @@
 namespace nlse {
 
 enum : int { PROP_SEP64 = 0, PROP_C64 = 1, PROP_SEP3D = 2, PROP_C128 = 3, PROP_KREAL = 4 };
+// compile-time kind of a column instance without a propagator (transforms only)
+constexpr int PK_NONE = 16;
 
 #ifndef NLSE_COL_PAIRED
 #define NLSE_COL_PAIRED 1
@@ -224,7 +226,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, int64_t b, float2* ld
 
   if (a.do_fwd) fft_line<N, P, G, false, GR, 3>(v, t, a.tw, lline, LINE);
   const int pkind = PK >= 0 ? PK : a.pkind;
-  if (a.do_prop) {
+  if (PK != PK_NONE && a.do_prop) {
     const int comp = (int)(slab % a.ncomp);
     if ((PK < 0 || PK == PROP_SEP64) && pkind == PROP_SEP64) {
       double2 px[G];
