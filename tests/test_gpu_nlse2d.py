@@ -202,3 +202,19 @@ def test_full_length_parity(gpu, precision):
     print(f"1001 steps {precision}: vs c64 oracle {err:.2e}, vs FP64 oracle {err64:.2e} "
           f"(c64 oracle vs FP64 {rel_l2(ref, ref64):.2e}), vs pocketfft32 c64 {errp:.2e}")
     assert err < PARITY
+
+
+def test_full_length_parity_1024(gpu):
+    """Config-0 physics over the full L = 10 mm (1001 steps) at 1024^2, where
+    the row sweep runs with 8 points per thread: the accumulated error stays
+    within the parity bound against the complex64 oracle."""
+    from nlse_amd import NLSE
+
+    N, z = 1024, 10e-3
+    sim = config0(NLSE, N)
+    E0 = config0_field(sim)
+    out = sim.out_field(E0, z, verbose=False)
+    ref = config0(oracle.NLSE, N, mode="c64").out_field(E0, z)
+    err = rel_l2(out, ref)
+    print(f"1001 steps at 1024^2: vs c64 oracle {err:.2e}")
+    assert err < PARITY
