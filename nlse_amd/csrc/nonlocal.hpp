@@ -403,29 +403,50 @@ __device__ __forceinline__ void band_body(const BandArgs& b, int tile, int64_t s
   for (int u0 = 0; u0 < b.kr; u0 += b.kc) {
     const int kc = (b.kr - u0) < b.kc ? (b.kr - u0) : b.kc;
     __syncthreads();
-    // kernel chunk, transposed: kt[v][u - u0]
-    for (int e = tid; e < kw * kc; e += 64 * BAND_WAVES) {
-      const int v = e / kc, u = e - v * kc;
-      kt[e] = b.KT[(int64_t)v * b.kr + u0 + u];
-    }
-    // staged row i holds field row y0 - r + u0 + i; column j holds x0 - r + j
-    for (int base = 0; base < total; base += 8 * 64 * BAND_WAVES) {
-      float vals[8];
+    // kernel chunk, transposed: kt[v][u - u0]; four loads in flight per
+    // thread before the stores
+    for (int e0 = tid; e0 < kw * kc; e0 += 4 * 64 * BAND_WAVES) {
+      float kv[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int e = base + k * 64 * BAND_WAVES + tid;
-        const int i = e / wcols, j = e - i * wcols;
-        const int y = tl.y0 - b.r + u0 + i, x = tl.x0 - b.r + j;
-        vals[k] = (e < total && y >= 0 && y < b.ny && x >= 0 && x < b.nx) ? sqmag(a[(int64_t)y * b.nx + x]) : 0.f;
+      for (int k = 0; k < 4; ++k) {
+        const int e = e0 + k * 64 * BAND_WAVES;
+        const int ec = e < kw * kc ? e : 0;
+        const int v = ec / kc, u = ec - v * kc;
+        kv[k] = b.KT[(int64_t)v * b.kr + u0 + u];
       }
 #pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (e0 + k * 64 * BAND_WAVES < kw * kc) kt[e0 + k * 64 * BAND_WAVES] = kv[k];
+    }
+    // staged row i holds field row y0 - r + u0 + i; column j holds x0 - r + j.
+    // Eight loads per thread are issued back to back: out-of-range points
+    // read element 0 and are zeroed after the load (a load under a branch
+    // makes the compiler wait for it at the join), and (i, j) advance
+    // incrementally instead of dividing by the tile width per element.
+    constexpr int NTH = 64 * BAND_WAVES;
+    const int di = NTH / wcols, dj = NTH - di * wcols;
+    for (int base = 0; base < total; base += 8 * NTH) {
+      const int e0 = base + tid;
+      int i = e0 / wcols, j = e0 - i * wcols;
+      float2 q[8];
+      int at[8];
+      bool ok[8];
+#pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int e = base + k * 64 * BAND_WAVES + tid;
-        if (e < total) {
-          const int i = e / wcols;
-          img[i * pitch + (e - i * wcols)] = vals[k];
+        const int y = tl.y0 - b.r + u0 + i, x = tl.x0 - b.r + j;
+        ok[k] = e0 + k * NTH < total && y >= 0 && y < b.ny && x >= 0 && x < b.nx;
+        q[k] = a[ok[k] ? (int64_t)y * b.nx + x : 0];
+        at[k] = i * pitch + j;
+        i += di;
+        j += dj;
+        if (j >= wcols) {
+          j -= wcols;
+          ++i;
         }
       }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (e0 + k * NTH < total) img[at[k]] = ok[k] ? sqmag(q[k]) : 0.f;
     }
     __syncthreads();
     if (lane_ok) {
