@@ -138,6 +138,7 @@ def main() -> int:
                     help="k0: the reference's K0 convolution; lorentzian: 1/(1+(k l)^2) in Fourier space")
     ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end out_field timings")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -266,12 +267,57 @@ def bench_c1(args) -> int:
         "grid_point_steps_per_s": steps_per_s * P,
         "field_finite": finite,
     }
+    if not args.no_e2e:
+        result["end_to_end"] = end_to_end(N, nl_length, lorentzian)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds, threads, nl_length,
                                               "lorentzian" if lorentzian else "k0")
     print(json.dumps(result), flush=True)
     return 0
+
+
+def end_to_end(N: int, nl_length: float, lorentzian: bool) -> dict:
+    """Wall time of whole out_field calls, host (numpy) field in and out, the
+    way the reference's published figure times them (propagator, plans,
+    transfers and normalisation included): the config-1 run over its full
+    length, and the settings of the reference's own 2-D benchmark
+    (examples/fig1_benchmarks.py: L = 2 mm, dz = 1e-4, 20 steps, V = None,
+    alpha = 22, Isat = 3.92e4, n2 = -1.99e-9, P = 1.05 W, window 11.3 mm,
+    mean of two calls on one instance),
+    which the reference reports at about 0.38 s at 2048^2 on an RTX 4090."""
+    import torch
+
+    from nlse_amd import NLSE
+    from nlse_amd import workloads as W
+    from nlse_amd.nlse import count_steps
+
+    sim = W.config1(NLSE, N, nonlocal_length=nl_length)
+    if lorentzian:
+        sim.nonlocal_mode = "lorentzian"
+    E0 = W.config1_field(sim)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = sim.out_field(E0, sim.L, verbose=False)
+    t_full = time.perf_counter() - t0
+    n_full = count_steps(sim.L, sim.delta_z)
+    # examples/fig1_benchmarks.py:17-59: window 3008 x 3.76 um, waist 2.29 mm,
+    # I_sat set after construction, the propagator built by the first call
+    ref = NLSE(alpha=22, power=1.05, window=3008 * 3.76e-6, n2=-1.99e-9, V=None, L=2e-3, NX=N, NY=N)
+    ref.I_sat = 3.92e4
+    ref.delta_z = 1e-4
+    G = np.exp(-(np.hypot(ref.XX, ref.YY) ** 2) / 2.29e-3**2).astype(np.complex64)
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        ref.out_field(G, 2e-3, verbose=False)
+        times.append(time.perf_counter() - t0)
+    return {
+        "config1_full_length": {"seconds": t_full, "steps": n_full, "steps_per_s_incl_overhead": n_full / t_full,
+                                "finite": bool(np.isfinite(out).all())},
+        "reference_fig1_settings": {"seconds_mean_of_2": float(np.mean(times)), "steps": count_steps(2e-3, 1e-4),
+                                    "published_rtx4090_seconds": 0.38 if N == 2048 else None},
+    }
 
 
 def _timed(fn, *a) -> float:
