@@ -256,10 +256,15 @@ int nlse_plan_set_nonlocal_spectrum(nlse_plan* plan, const double* quad, int64_t
 int nlse_run(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t step0,
              void* stream);
 /* nlse_run for a field that stays at the same address: the steady steps
- * are captured once into a hipGraph of `chunk` steps and replayed (the
- * capture is kept until a table changes or the field moves).  Results are
- * identical to nlse_run.  Plans with a drive profile and 1-D plans (one
- * launch per run) run as nlse_run. */
+ * are captured once into a hipGraph of `chunk` steps and replayed.  The
+ * capture is kept while the field address, the table pointers and the
+ * table kinds stay the same: setters that rewrite a table in place
+ * (nonlinear records, Rabi, propagator factors, V contents) keep it, and
+ * the replay reads the new values; the nonlocal and drive setters drop it.
+ * Setters copy synchronously and must not be called while a run of the
+ * plan is in flight on another stream.  Results are identical to nlse_run.
+ * Plans with a drive profile and 1-D plans (one launch per run) run as
+ * nlse_run. */
 int nlse_run_graph(nlse_plan* plan, void* A_dev, int64_t nsteps, int precision, int64_t chunk, void* stream);
 /* Launch one sweep type `count` times in place on A (device): pass 0 is a
  * steady-state row sweep (I_x, nonlinear step, F_x), pass 1 a column sweep
