@@ -101,16 +101,18 @@ def kernel_bytes(P: int, N: int, r: int, local: bool, lorentzian: bool = False) 
     points, complex64 field, float32 V).  Nonlocal one-component fields use
     the packed intensity torus of P/2 complex points (4 B per field point)
     and band points within r of its edges; the Fourier-space (Lorentzian)
-    response keeps one complex intensity line per row (8 B per point)."""
+    response packs rows the same way and has no band points."""
     if local:
         return {"row": (0, P * (16 + 4)), "col": (1, P * 16)}
     if lorentzian:
+        # row pairs packed into one complex intensity line (4 B per field
+        # point), the full-torus spectrum quadrant
         khat = 4 * (N // 2 + 1) * (N // 2 + 1)
         return {
             "col": (1, P * 16),
-            "nlc_z": (2, P * (8 + 8 + 8)),
-            "colZ": (4, P * 16 + khat),
-            "nlc_nl": (5, P * (8 + 8 + 4 + 8)),
+            "nlc_z": (2, P * (8 + 8 + 4)),
+            "colZ": (4, P * 8 + khat),
+            "nlc_nl": (5, P * (8 + 4 + 4 + 8)),
         }
     band = 4 * r * N + (N - 4 * r) * 2 * r
     khat = 4 * (N // 4 + 1) * (N // 2 + 1)

@@ -551,6 +551,7 @@ struct nlse_plan {
     int r = 0;                      // kernel half-width (after trimming)
     int64_t M = 0;                  // row pairing of the packed intensity (torus rows unless full)
     bool full = false;              // one component on the full NY-row torus (spectral response)
+    bool unpack = false;            // spectral response of row-pair-packed spectra (k_colu)
     bool direct = false;            // every point convolved directly
     const LineOps* zops = nullptr;  // columns of length M
     const float4* twz = nullptr;
@@ -909,7 +910,16 @@ int nlse_plan_set_nonlocal_spectrum(nlse_plan* p, const double* quad, int64_t qy
   int rc = NLSE_OK;
   n.r = 0;
   n.direct = false;
-  n.full = p->d.ncomp == 1;
+  // one component: rows packed in pairs (y, y + NY/2) as for the kernel
+  // convolution, the full-torus y convolution done on unpacked column
+  // pairs (k_colu); two components: one complex line I1 + i I2 per row.
+  // NLSE_SPECTRAL_FULL_ROWS selects the earlier form that transforms every
+  // row's intensity on its own (twice the intensity work).
+#ifndef NLSE_SPECTRAL_FULL_ROWS
+#define NLSE_SPECTRAL_FULL_ROWS 0
+#endif
+  n.full = p->d.ncomp == 1 && NLSE_SPECTRAL_FULL_ROWS;
+  n.unpack = p->d.ncomp == 1 && !NLSE_SPECTRAL_FULL_ROWS;
   n.M = p->d.ncomp == 1 ? ny / 2 : ny;
   n.nb = 0;
   std::vector<int> rowmap(ny, -1);
@@ -921,7 +931,8 @@ int nlse_plan_set_nonlocal_spectrum(nlse_plan* p, const double* quad, int64_t qy
   if (!rc) rc = n.zops->twiddles(&n.twz);
   if (!rc) rc = upload((void**)&n.rowmap, rowmap.data(), rowmap.size() * sizeof(int));
   if (!rc) rc = upload((void**)&n.kq, kq.data(), kq.size() * sizeof(float));
-  if (!rc && hipMalloc((void**)&n.Z, sizeof(float2) * items * ny * nx) != hipSuccess)
+  const int64_t zrows = n.full ? ny : n.M;
+  if (!rc && hipMalloc((void**)&n.Z, sizeof(float2) * items * zrows * nx) != hipSuccess)
     rc = fail(NLSE_ERR_HIP, "hipMalloc of the intensity buffer failed");
   if (rc) {
     nlc_free(p);
@@ -1111,6 +1122,7 @@ static int band_and_z(nlse_plan* p, const void* A, hipStream_t s, const ColArgs*
   auto& n = p->nlc;
   const int64_t slabs = p->d.items * p->d.ncomp;
   const ColArgs cz = cz_override ? *cz_override : nlc_col_args(p);
+  if (n.unpack) return n.zops->colu(cz, s);  // spectral response: no band points
   int rc = NLSE_OK;
   if (NLSE_MERGE_BAND && !n.direct && n.ntiles > 0 && slabs <= 65535) {
     rc = n.zops->colz_band(n.colw, cz, band_args(p, A), n.pitch, n.lds, (int64_t)n.ntiles * slabs, s);
@@ -1383,7 +1395,7 @@ int nlse_run_pass(nlse_plan* p, void* A, int pass, int64_t count, int precision,
     for (int64_t k = 0; k < count && !rc; ++k) {
       if (pass == 2) rc = p->rops->nlc(1, a, s);
       else if (pass == 3) rc = band_launch(p, A, s);
-      else if (pass == 4) rc = p->nlc.zops->col(p->nlc.colw, cz, s);
+      else if (pass == 4) rc = p->nlc.unpack ? p->nlc.zops->colu(cz, s) : p->nlc.zops->col(p->nlc.colw, cz, s);
       else if (pass == 6) rc = band_and_z(p, A, s, &cz);
       else rc = p->rops->nlc(2, a, s);
     }

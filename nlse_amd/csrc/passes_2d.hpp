@@ -322,6 +322,69 @@ __global__ __launch_bounds__(WT*(N / P)) void k_col(ColArgs a) {
   col_body<N, P, WT, G, GR, PK>(a, b, lds);
 }
 
+// Intensity convolution over the full NY-row torus (a response given by
+// its spectrum) for one-component fields whose intensity rows are packed
+// in pairs (p, p + M), M = NY/2, along x:  Z(p, kx) = F_x(I_p + i I_{p+M}).
+// Unit j in [0, NX/2] owns the columns a = j and b = (NX - j) mod NX: it
+// unpacks the spectra of the two real rows,
+//   X_p(j) = (Z(p,a) + conj Z(p,b)) / 2,   X_{p+M}(j) = (Z(p,a) - conj Z(p,b)) / 2i,
+// into one NY-point column (a thread's set m < P/2 holds rows p = t + mT,
+// m >= P/2 the rows p + M), applies I_y K^ F_y, and packs the result back
+// into both columns (the convolved rows are real, so their spectra at -j
+// are the conjugates).  Half the column work of transforming every row's
+// intensity on its own.  Units past NX/2 in the last tile compute unit 0
+// again without storing (every thread passes the barriers).
+template <int N, int P, int WT>
+__global__ __launch_bounds__(WT*(N / P)) void k_colu(ColArgs a) {
+  using PL = Plan<N, P>;
+  constexpr int T = PL::T;
+  constexpr int LINE = LdsLine<N, 3, 3>::SIZE;
+  constexpr int H = P / 2;
+  static_assert(H * T == N / 2, "the first half of a natural set covers rows [0, N/2)");
+  __shared__ float2 lds[col_lds_size<N, WT, 1>()];
+  const int units = a.NX / 2 + 1;
+  const int tiles = (units + WT - 1) / WT;
+  const int64_t nblk = (int64_t)gridDim.x;
+  const int64_t bb = (nblk % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int64_t slab = bb / tiles;
+  const int cw = threadIdx.x % WT;
+  const int t = threadIdx.x / WT;
+  const int j0 = (int)(bb % tiles) * WT + cw;
+  const bool act = j0 < units;
+  const int j = act ? j0 : 0;
+  const int ca = j, cb = j == 0 ? 0 : a.NX - j;
+  float2* Z = a.A + slab * (int64_t)(N / 2) * a.NX;
+  float2* lline = lds + cw * LINE;
+
+  float2 v[1][P];
+#pragma unroll
+  for (int m = 0; m < H; ++m) {
+    const int64_t row = (int64_t)(t + m * T) * a.NX;
+    const float2 za = Z[row + ca], zb = Z[row + cb];
+    v[0][m] = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+    v[0][m + H] = make_float2(0.5f * (za.y + zb.y), 0.5f * (zb.x - za.x));
+  }
+  fft_line<N, P, 1, false, 1, 3>(v, t, a.tw, lline, LINE);
+  const int hx = a.NX / 2 + 1;
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int ky = t + m * T;
+    const int kyq = ky <= N / 2 ? ky : N - ky;
+    const float f = a.kq[kyq * hx + j] * a.scale;
+    v[0][m] = make_float2(v[0][m].x * f, v[0][m].y * f);
+  }
+  fft_line<N, P, 1, true, 1, 3>(v, t, a.tw, lline, LINE);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+      const int64_t row = (int64_t)(t + m * T) * a.NX;
+      const float2 yt = v[0][m], yb = v[0][m + H];
+      Z[row + ca] = make_float2(yt.x - yb.y, yt.y + yb.x);
+      if (cb != ca) Z[row + cb] = make_float2(yt.x + yb.y, yb.x - yt.y);
+    }
+  }
+}
+
 // Intensity-spectrum column sweep (PROP_KREAL) and the band pass of the
 // nonlocal response in one launch: blocks [0, ncol) run column tiles of W =
 // WT*G columns (one LDS round per line), the others run blockDim/256 band
