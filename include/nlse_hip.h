@@ -150,6 +150,16 @@ int nlse_delta_n_max(const void* A_dev, int64_t n_items, int64_t item_size, floa
 int nlse_vortex(void* im_dev, int im_kind, int64_t n, double i0, double j0, const void* ii_dev,
                 const void* jj_dev, int coord_kind, int charge, void* stream);
 
+/* Dense propagator table of NLSE._build_propagator (nlse.py:188-197) for an
+ * (ny, nx) grid, complex64:
+ *   out[y*nx + x] = complex64(exp(-1j * 0.5 * (kx[x]^2 + ky[y]^2) / k * dz))
+ * from the FP64 wavenumbers kx_dev[nx], ky_dev[ny] and inv_k = 1.0 / k,
+ * evaluated in FP64 in numpy's operation order (theta = ((-0.5 s) * inv_k)
+ * * dz, each operation rounded, then cos/sin) and rounded to complex64 —
+ * equal to the reference's host table. */
+int nlse_propagator_dense(void* out_dev, const double* kx_dev, const double* ky_dev, int64_t nx, int64_t ny,
+                          double inv_k, double dz, void* stream);
+
 /* Counter-based random numbers (no counterpart in the reference, which
  * draws from cupy.random / numpy.random): Philox4x64-10, the generator of
  * numpy.random.Philox.  Block b (b < nblocks) encrypts the counter

@@ -77,6 +77,7 @@ SIGNATURES = {
     "nlse_delta_n_max": (c_int, [c_void_p, c_i64, c_i64, C.c_float, C.c_float, BParam, BParam, c_void_p,
                                  c_void_p]),
     "nlse_vortex": (c_int, [c_void_p, c_int, c_i64, c_double, c_double, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "nlse_propagator_dense": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_double, c_double, c_void_p]),
     "nlse_philox4x64": (c_int, [c_void_p, c_i64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), c_void_p]),
     "nlse_add_noise": (c_int, [c_void_p, c_i64, c_i64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), c_double,
                                c_double, c_double, c_void_p]),

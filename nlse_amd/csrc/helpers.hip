@@ -1,6 +1,7 @@
 // Initial-state and drive helpers on the device: the vortex phase imprint of
-// kernels_gpu.py:159-176 and the noise of DDGPE.add_noise (ddgpe.py:96-125)
-// from a counter-based generator (Philox4x64-10).
+// kernels_gpu.py:159-176, the noise of DDGPE.add_noise (ddgpe.py:96-125)
+// from a counter-based generator (Philox4x64-10) and the dense 2-D
+// propagator table of nlse.py:188-197.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -150,6 +151,26 @@ __global__ void k_add_noise(float2* A, int64_t items, int64_t isz, Ctr ctr, uint
   }
 }
 
+// ------------------------------------------------------------ propagator
+// Dense table of nlse.py:188-197, exp(-1j * 0.5 * (Kxx**2 + Kyy**2) / k * dz)
+// cast to complex64, with numpy's FP64 operation order: the product with
+// -0.5j leaves a zero real part and an imaginary part -0.5 S exactly, the
+// complex division by the real k multiplies by 1/k (numpy's division loop
+// scales by the reciprocal of the divisor), the product with dz is one
+// rounding, and exp of (0 + i theta) is (cos theta, sin theta).
+__global__ void k_prop_dense(float2* out, const double* kx, const double* ky, int64_t nx, int64_t n, double inv_k,
+                             double dz) {
+#pragma clang fp contract(off)  // every product and sum rounds as numpy's do
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const double x = kx[p % nx], y = ky[p / nx];
+    const double s = x * x + y * y;
+    const double theta = ((-0.5 * s) * inv_k) * dz;
+    double sn, cs;
+    sincos(theta, &sn, &cs);
+    out[p] = make_float2((float)cs, (float)sn);
+  }
+}
+
 bool num_kind_ok(int k) { return k == NLSE_NUM_F32 || k == NLSE_NUM_F64 || k == NLSE_NUM_I32 || k == NLSE_NUM_I64; }
 
 }  // namespace
@@ -165,6 +186,16 @@ int nlse_vortex(void* im, int im_kind, int64_t n, double i0, double j0, const vo
   hipLaunchKernelGGL(k_vortex, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, im, im_kind, n, i0, j0, ii, jj,
                      coord_kind, charge);
   return launch_status("k_vortex");
+}
+
+int nlse_propagator_dense(void* out, const double* kx, const double* ky, int64_t nx, int64_t ny, double inv_k,
+                          double dz, void* stream) {
+  if (!out || !kx || !ky || nx < 0 || ny < 0) return fail(NLSE_ERR_ARG, "bad propagator arguments");
+  if (nx == 0 || ny == 0) return NLSE_OK;
+  const int64_t n = nx * ny;
+  hipLaunchKernelGGL(k_prop_dense, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)out, kx, ky, nx, n,
+                     inv_k, dz);
+  return launch_status("k_prop_dense");
 }
 
 int nlse_philox4x64(uint64_t* out, int64_t nblocks, const uint64_t* counter, const uint64_t* key, void* stream) {

@@ -238,6 +238,18 @@ def vortex_cp(im: torch.Tensor, i, j, ii: torch.Tensor, jj: torch.Tensor, ll: in
                                        int(ll), D.stream_ptr()), "vortex")
 
 
+def propagator_dense(kx, ky, k: float, dz: float) -> torch.Tensor:
+    """complex64 (len(ky), len(kx)) device table exp(-1j * 0.5 * (kx^2 + ky^2)
+    / k * dz) of nlse.py:188-197, equal to the reference's numpy evaluation
+    (see nlse_propagator_dense)."""
+    kx = D.to_device(np.asarray(kx, dtype=np.float64).reshape(-1))
+    ky = D.to_device(np.asarray(ky, dtype=np.float64).reshape(-1))
+    out = torch.empty((ky.numel(), kx.numel()), dtype=torch.complex64, device=D.device())
+    _lib.check(_lib.load().nlse_propagator_dense(out.data_ptr(), kx.data_ptr(), ky.data_ptr(), kx.numel(), ky.numel(),
+                                                 1.0 / float(k), float(dz), D.stream_ptr()), "propagator_dense")
+    return out
+
+
 def _u64(values, n: int):
     import ctypes as C
 

@@ -172,8 +172,11 @@ class NLSE:
         self._prop_factors = None
 
     def _build_propagator(self) -> np.ndarray:
-        """nlse.py:188-197: complex64, built in FP64 on the host."""
-        return np.exp(-1j * 0.5 * (self.Kxx**2 + self.Kyy**2) / self.k * self.delta_z).astype(np.complex64)
+        """nlse.py:188-197: exp(-1j * 0.5 * (Kxx**2 + Kyy**2) / k * delta_z) as
+        complex64, evaluated on the device in FP64 in numpy's operation order
+        (nlse_propagator_dense) and returned as a host array equal to the
+        reference's."""
+        return kernels_hip.propagator_dense(self.Kx, self.Ky, self.k, self.delta_z).cpu().numpy()
 
     def _separable_factors(self):
         """(px, py) complex128 with propagator = complex64(px[None,:]*py[:,None])
@@ -243,14 +246,24 @@ class NLSE:
             self._scale_items(A, scale, max(1, int(np.prod(lead))), isz, isz)
         return A, A_sq
 
+    def _separable_in_use(self) -> bool:
+        """The engine applies the propagator from its FP64 separable factors
+        (the dense table is this instance's own, unchanged since built)."""
+        return self._prop_factors is not None and self._prop_factors[0] is self._propagator
+
     def _send_arrays_to_gpu(self) -> None:
         """nlse.py:309-327: V, nl_profile, propagator and array parameters
-        become device tensors for the duration of out_field."""
+        become device tensors for the duration of out_field.  Two arrays
+        the engine never reads on the device stay on the host (their values
+        are those a round trip would give back): the unit nl_profile of a
+        local response (nl_length = 0, NY x NX float32) and the dense
+        propagator table when the engine applies its separable factors."""
         if self.V is not None and not D.is_device_tensor(self.V):
             self.V = D.to_device(self.V)
-        if not D.is_device_tensor(self.nl_profile):
+        if self.nl_length > 0 and not D.is_device_tensor(self.nl_profile):
             self.nl_profile = D.to_device(self.nl_profile)
-        if self._propagator is not None and not D.is_device_tensor(self._propagator):
+        if (self._propagator is not None and not self._separable_in_use()
+                and not D.is_device_tensor(self._propagator)):
             self._set_propagator_internal(D.to_device(self._propagator))
         for name in ("power", "n2", "alpha", "I_sat"):
             val = getattr(self, name)

@@ -177,3 +177,18 @@ def test_add_noise_matches_oracle_and_statistics(gpu):
     amp = 2.0 * np.sqrt(sim.gamma / (4 * sim.delta_X * sim.delta_Y))
     assert abs(np.std(first[0].real) / (amp * sim.delta_z) - 1) < 0.03
     assert rel_l2(second, first) > 1.0  # fresh draw, not a repeat
+
+
+@pytest.mark.parametrize("ny,nx,dz", [(2048, 2048, 5e-6), (128, 256, 1e-4), (64, 64, 3.3e-3)])
+def test_propagator_table_equals_numpy(gpu, ny, nx, dz):
+    """NLSE._build_propagator builds the dense table on the device: equal,
+    value for value, to the reference's numpy expression (nlse.py:188-197)."""
+    from nlse_amd import NLSE
+
+    s = NLSE(alpha=20.0, power=1.05, window=[nx * 3.76e-6, ny * 3.76e-6], n2=-1.6e-9, V=None, L=1e-3, NX=nx,
+             NY=ny)
+    s.delta_z = dz
+    got = s._build_propagator()
+    want = np.exp(-1j * 0.5 * (s.Kxx**2 + s.Kyy**2) / s.k * s.delta_z).astype(np.complex64)
+    assert got.dtype == np.complex64 and got.shape == (ny, nx)
+    assert np.array_equal(got, want), np.count_nonzero(got != want)
