@@ -1,9 +1,11 @@
-"""Parameter host<->device round trips (test_nlse.py:178-240,
-test_cnlse.py:91-185, test_ddgpe.py:85-200): after _send_arrays_to_gpu the
-propagator, V and the per-realisation array parameters are device tensors,
-after _retrieve_arrays_from_gpu numpy arrays again, with unchanged values.
-An out_field call leaves the instance's own propagator as a host array (the
-engine applies its separable factors and never moves the dense table)."""
+"""Host-side API of the reference tests: parameter host<->device round
+trips (test_nlse.py:178-240, test_cnlse.py:91-185, test_ddgpe.py:85-200):
+after _send_arrays_to_gpu the propagator, V and the per-realisation array
+parameters are device tensors, after _retrieve_arrays_from_gpu numpy arrays
+again, with unchanged values; an out_field call leaves the instance's own
+propagator as a host array (the engine applies its separable factors and
+never moves the dense table); FFT plan type and shape (test_nlse.py:53-99,
+test_nlse3d.py:62-103)."""
 
 import numpy as np
 import pytest
@@ -93,3 +95,26 @@ def test_out_field_keeps_own_propagator_on_host(gpu):
     out3 = simu.out_field(E0, 5e-4, verbose=False)
     assert isinstance(simu.propagator, np.ndarray) and np.array_equal(simu.propagator, want)
     assert np.abs(out3 - out1).max() <= 1e-6 * np.abs(out1).max()
+
+
+def test_build_fft_plan(gpu):
+    """test_nlse.py:53-99 and test_nlse3d.py:62-103 (GPU branch): one plan
+    over the field's shape, here a HipFFTPlan (the VkFFTApp stand-in), and it
+    transforms: forward then inverse returns the field."""
+    from nlse_amd import NLSE, NLSE_3d, NLSE_1d
+    from nlse_amd.fft import HipFFTPlan
+
+    rng = np.random.default_rng(4)
+    s2 = NLSE(20, 1.05, window, -1.6e-9, None, 10e-3, NX=N, NY=N, Isat=1e5)
+    s3 = NLSE_3d(alpha=20, energy=1e-6, window=[window, window, 2e-12], n2=-1.6e-9, D0=1e-26, vg=1.0, V=None,
+                 L=1e-3, NX=32, NY=32, NZ=16, Isat=1e5)
+    s1 = NLSE_1d(20, 1.05, window, -1.6e-9, None, 10e-3, NX=N, Isat=1e5)
+    for simu, shape in ((s2, (N, N)), (s3, (32, 32, 16)), (s1, (3, N))):
+        A = torch.from_numpy((rng.random(shape) + 1j * rng.random(shape)).astype(np.complex64)).cuda()
+        plans = simu._build_fft_plan(A)
+        assert len(plans) == 1 and isinstance(plans[0], HipFFTPlan)
+        assert plans[0].shape0 == shape
+        B = A.clone()
+        plans[0].fft(B, B)
+        plans[0].ifft(B, B)
+        assert torch.allclose(B, A, atol=1e-5)
