@@ -5,7 +5,8 @@ parameters are device tensors, after _retrieve_arrays_from_gpu numpy arrays
 again, with unchanged values; an out_field call leaves the instance's own
 propagator as a host array (the engine applies its separable factors and
 never moves the dense table); FFT plan type and shape (test_nlse.py:53-99,
-test_nlse3d.py:62-103)."""
+test_nlse3d.py:62-103); _prepare_output_array of every class against the
+oracle class."""
 
 import numpy as np
 import pytest
@@ -118,3 +119,45 @@ def test_build_fft_plan(gpu):
         plans[0].fft(B, B)
         plans[0].ifft(B, B)
         assert torch.allclose(B, A, atol=1e-5)
+
+
+def _classes():
+    from scipy.constants import atomic_mass
+
+    h = 0.654
+    return [
+        ("NLSE_1d", dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, V=None, L=1e-3, NX=256, Isat=1e5), (3, 256)),
+        ("NLSE_3d", dict(alpha=20, energy=1e-6, window=[window, window, 2e-12], n2=-1.6e-9, D0=1e-26, vg=1.0, V=None,
+                         L=1e-3, NX=32, NY=32, NZ=16, Isat=1e5), (32, 32, 16)),
+        ("GPE", dict(gamma=0, N=1e6, window=1e-3, g=1e-6, V=None, m=87 * atomic_mass, NX=64, NY=64), (2, 64, 64)),
+        ("CNLSE", dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, n12=-1e-10, V=None, L=1e-3, NX=64, NY=64,
+                       Isat=1e5), (2, 64, 64)),
+        ("CNLSE_1d", dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, n12=-1e-10, V=None, L=1e-3, NX=256,
+                          Isat=1e5), (3, 2, 256)),
+        ("DDGPE", dict(gamma=0.0, power=17.0, window=256, g=1e-2 / h, omega=5.07 / h, T=1, omega_exc=1484.44 / h,
+                       omega_cav=1482.76 / h, detuning=0.17 / h, k_z=27, NX=64, NY=64), (2, 64, 64)),
+    ]
+
+
+@pytest.mark.parametrize("name,kw,shape", _classes(), ids=[c[0] for c in _classes()])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_prepare_output_array_all_classes(gpu, name, kw, shape, normalize):
+    """test_nlse_1d.py:39-94, test_nlse3d.py:106-173, test_gpe.py:52-104,
+    test_cnlse.py:31-88, test_cnlse_1d.py:51-114, test_ddgpe.py:32-82:
+    contiguous outputs of the input's shape, normalised as the class
+    normalises (equal to the oracle class, whose normalisations are pinned
+    to their integrals)."""
+    import nlse_amd
+    import oracle
+
+    rng = np.random.default_rng(5)
+    E = (rng.random(shape) + 1j * rng.random(shape)).astype(np.complex64)
+    g = getattr(nlse_amd, name)(**kw)
+    o = getattr(oracle, name)(**kw)
+    out, out_sq = g._prepare_output_array(E, normalize)
+    ref, _ = o._prepare_output_array(E, normalize)
+    assert out.is_contiguous() and out_sq.is_contiguous()
+    assert tuple(out.shape) == shape and tuple(out_sq.shape) == shape
+    got = out.cpu().numpy().astype(np.complex128)
+    ref = np.asarray(ref).astype(np.complex128)
+    assert np.linalg.norm(got - ref) <= 1e-6 * np.linalg.norm(ref)
