@@ -244,3 +244,29 @@ def test_ddgpe_matches_oracle(gpu, with_sampler):
     assert np.abs(ref[1]).max() > 0 and err < PARITY
     if with_sampler:
         assert samples[nsteps - 1] > 0
+
+
+@pytest.mark.parametrize("name,shape", [("NLSE_1d", (256,)), ("CNLSE", (2, 64, 64)), ("CNLSE_1d", (2, 256))])
+def test_one_step_identity(gpu, name, shape):
+    """test_nlse_1d.py:96-121, test_cnlse.py:220-259, test_cnlse_1d.py:116-150:
+    delta_z = 0, a field of ones, one 'double' split step returns ones
+    (asserted on the stepped field; the reference's CNLSE variants assert on
+    the input)."""
+    import nlse_amd
+
+    kw = dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, V=None, L=10e-3, Isat=1e5)
+    if name.startswith("CNLSE"):
+        kw["n12"] = -1e-10
+    if name.endswith("_1d"):
+        kw["NX"] = shape[-1]
+    else:
+        kw.update(NX=shape[-1], NY=shape[-2])
+    simu = getattr(nlse_amd, name)(**kw)
+    simu.delta_z = 0
+    simu.propagator = simu._build_propagator()
+    A, A_sq = simu._prepare_output_array(np.ones(shape, np.complex64), normalize=False)
+    simu.plans = simu._build_fft_plan(A)
+    simu._send_arrays_to_gpu()
+    simu.split_step(A, A_sq, simu.V, simu.propagator, simu.plans, precision="double")
+    simu._retrieve_arrays_from_gpu()
+    assert torch.allclose(A, torch.ones_like(A))
