@@ -144,8 +144,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every GPU box here has one "
-              "GPU; the multi-GPU design is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one process "
+              "on one GPU; sharding across GPUs is not built).", file=sys.stderr)
         return 2
     defaults = {"c1": (1000, 50), "c2": (500, 20), "c3": (200, 10), "c4": (100, 5)}[args.workload]
     args.steps = defaults[0] if args.steps is None else args.steps

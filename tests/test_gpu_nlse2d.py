@@ -12,8 +12,8 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-# Parity bound of the north star: relative L2 <= 1e-5 against the complex64
-# reference-CPU oracle.
+# Parity bound: relative L2 <= 1e-5 against the complex64 reference-CPU
+# oracle.
 PARITY = 1e-5
 
 waist = 2.23e-3

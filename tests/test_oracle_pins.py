@@ -34,7 +34,7 @@ def small_nlse(N=64, **kw):
                                     (2e-3, 1e-4, 20)])
 def test_step_count_rule(z, dz, n):
     """The loop `while abs(z_prop) < z: z_prop += dz` (nlse.py:570,584): the
-    float accumulation decides the count (the SURVEY table)."""
+    float accumulation decides the count."""
     assert oracle.count_steps(z, dz) == n
 
 
