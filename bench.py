@@ -4,7 +4,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--local]
 
 A step is one split step (linear step in Fourier space + real-space
-nonlinear step) of the 2048x2048 complex64 field of the config-1 workload:
+nonlinear step) of the 2048x2048 complex64 field of the speckle workload:
 alpha=20, n2=-1.6e-9, Isat=1e5, real potential V, dz=5e-6, speckle input,
 and the nonlocal response of nl_length = 20 um (the reference's K0 kernel:
 d = nl_length // dX = 4 points, a 25x25 kernel applied as a zero-padded
@@ -68,10 +68,10 @@ def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float, mode: 
     import oracle
     from nlse_amd import workloads as W
 
-    sim = W.config1(oracle.NLSE, N, nonlocal_length=nl_length, fft="pocketfft32", workers=threads,
+    sim = W.speckle_nonlocal_nlse(oracle.NLSE, N, nonlocal_length=nl_length, fft="pocketfft32", workers=threads,
                     threads=threads)
     sim.nonlocal_mode = mode
-    E0 = W.config1_field(sim)
+    E0 = W.speckle_field(sim)
     A, A_sq = sim._prepare_output_array(E0, True)
     prop = sim._build_propagator()
     V = np.array(sim.V, copy=True)
@@ -88,7 +88,7 @@ def cpu_baseline(N: int, budget_s: float, threads: int, nl_length: float, mode: 
         "unit": "split-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} split steps of the {N}x{N} config-1 workload "
+        "sample": f"{n} split steps of the {N}x{N} speckle workload "
                   f"({('nonlocal, ' + ('FP64 FFT Lorentzian' if mode == 'lorentzian' else 'scipy oaconvolve')) if nl_length > 0 else 'without the nonlocal term'}) on the "
                   f"CPU oracle: complex64, scipy.fft single precision with {threads} workers, elementwise "
                   f"steps on {threads} threads; {el:.1f} s; host CPU: {cpu_model()} "
@@ -129,13 +129,13 @@ def kernel_bytes(P: int, N: int, r: int, local: bool, lorentzian: bool = False) 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4"), default="c1",
-                    help="c1: 2048^2 nonlocal split step (the metric); c2: uneven batch; c3: 3-D with GVD; "
-                         "c4: learned corrector")
+    ap.add_argument("--workload", choices=("speckle", "batch", "gvd3d", "corrector"), default="speckle",
+                    help="speckle: 2048^2 nonlocal split step (the metric); batch: 64 uneven fields; "
+                         "gvd3d: 3-D 512x512x64 with GVD; corrector: learned corrector at 1024^2")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--grid", type=int, default=2048)
-    ap.add_argument("--local", action="store_true", help="drop the nonlocal term (config-1 physics otherwise)")
+    ap.add_argument("--local", action="store_true", help="drop the nonlocal term (speckle-workload physics otherwise)")
     ap.add_argument("--nonlocal-mode", choices=("k0", "lorentzian"), default="k0",
                     help="k0: the reference's K0 convolution; lorentzian: 1/(1+(k l)^2) in Fourier space")
     ap.add_argument("--col-width", type=int, default=0, help="columns per block of the column sweep (0: default)")
@@ -147,19 +147,19 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one process "
               "on one GPU; sharding across GPUs is not built).", file=sys.stderr)
         return 2
-    defaults = {"c1": (1000, 50), "c2": (500, 20), "c3": (200, 10), "c4": (100, 5)}[args.workload]
+    defaults = {"speckle": (1000, 50), "batch": (500, 20), "gvd3d": (200, 10), "corrector": (100, 5)}[args.workload]
     args.steps = defaults[0] if args.steps is None else args.steps
     args.warmup = defaults[1] if args.warmup is None else args.warmup
-    if args.workload == "c2":
-        return bench_c2(args)
-    if args.workload == "c3":
-        return bench_c3(args)
-    if args.workload == "c4":
-        return bench_c4(args)
-    return bench_c1(args)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "gvd3d":
+        return bench_gvd3d(args)
+    if args.workload == "corrector":
+        return bench_corrector(args)
+    return bench_speckle(args)
 
 
-def bench_c1(args) -> int:
+def bench_speckle(args) -> int:
     import torch
 
     from nlse_amd import NLSE
@@ -167,12 +167,12 @@ def bench_c1(args) -> int:
 
     N = args.grid
     nl_length = 0.0 if args.local else 20e-6
-    sim = W.config1(NLSE, N, nonlocal_length=nl_length)
+    sim = W.speckle_nonlocal_nlse(NLSE, N, nonlocal_length=nl_length)
     sim.col_width = args.col_width
     lorentzian = not args.local and args.nonlocal_mode == "lorentzian"
     if lorentzian:
         sim.nonlocal_mode = "lorentzian"
-    E0 = W.config1_field(sim)
+    E0 = W.speckle_field(sim)
     sim._own_propagator()
     sim._send_arrays_to_gpu()
     A, _ = sim._prepare_output_array(E0, True)
@@ -221,12 +221,12 @@ def bench_c1(args) -> int:
     physics = ("real float32 potential, Isat=1e5, alpha=20, dz=5e-6, speckle input (seed 1), "
                "precision='single'")
     if args.local:
-        workload = f"config-1 physics without the nonlocal term: {N}x{N} complex64 NLSE split step, " + physics
+        workload = f"speckle-workload physics without the nonlocal term: {N}x{N} complex64 NLSE split step, " + physics
     elif lorentzian:
-        workload = (f"config-1: {N}x{N} complex64 NLSE split step with the nonlocal response applied in "
+        workload = (f"{N}x{N} complex64 NLSE split step with the nonlocal response applied in "
                     "Fourier space as 1/(1+(k l)^2), l = 20 um (periodic), " + physics)
     else:
-        workload = (f"config-1: {N}x{N} complex64 NLSE split step with the nonlocal response "
+        workload = (f"{N}x{N} complex64 NLSE split step with the nonlocal response "
                     f"(nl_length 20 um, reference K0 kernel {2 * r + 1}x{2 * r + 1}, zero-padded linear "
                     "convolution), " + physics)
     result = {
@@ -282,7 +282,7 @@ def bench_c1(args) -> int:
 def end_to_end(N: int, nl_length: float, lorentzian: bool) -> dict:
     """Wall time of whole out_field calls, host (numpy) field in and out, the
     way the reference's published figure times them (propagator, plans,
-    transfers and normalisation included): the config-1 run over its full
+    transfers and normalisation included): the speckle-workload run over its full
     length, and the settings of the reference's own 2-D benchmark
     (examples/fig1_benchmarks.py: L = 2 mm, dz = 1e-4, 20 steps, V = None,
     alpha = 22, Isat = 3.92e4, n2 = -1.99e-9, P = 1.05 W, window 11.3 mm,
@@ -294,10 +294,10 @@ def end_to_end(N: int, nl_length: float, lorentzian: bool) -> dict:
     from nlse_amd import workloads as W
     from nlse_amd.nlse import count_steps
 
-    sim = W.config1(NLSE, N, nonlocal_length=nl_length)
+    sim = W.speckle_nonlocal_nlse(NLSE, N, nonlocal_length=nl_length)
     if lorentzian:
         sim.nonlocal_mode = "lorentzian"
-    E0 = W.config1_field(sim)
+    E0 = W.speckle_field(sim)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     out = sim.out_field(E0, sim.L, verbose=False)
@@ -315,7 +315,7 @@ def end_to_end(N: int, nl_length: float, lorentzian: bool) -> dict:
         ref.out_field(G, 2e-3, verbose=False)
         times.append(time.perf_counter() - t0)
     return {
-        "config1_full_length": {"seconds": t_full, "steps": n_full, "steps_per_s_incl_overhead": n_full / t_full,
+        "speckle_nonlocal_full_length": {"seconds": t_full, "steps": n_full, "steps_per_s_incl_overhead": n_full / t_full,
                                 "finite": bool(np.isfinite(out).all())},
         "reference_fig1_settings": {"seconds_mean_of_2": float(np.mean(times)), "steps": count_steps(2e-3, 1e-4),
                                     "published_rtx4090_seconds": 0.38 if N == 2048 else None},
@@ -337,7 +337,7 @@ def _timed(fn, *a) -> float:
     return e0.elapsed_time(e1) * 1e-3
 
 
-def bench_c2(args) -> int:
+def bench_batch(args) -> int:
     """Uneven batch: 64 fields of 256^2/512^2/1024^2 with random n2, power,
     Isat, one persistent launch per sweep for the whole batch, steady steps
     replayed from a captured graph.  Compared with the same fields run one
@@ -347,7 +347,7 @@ def bench_c2(args) -> int:
     from nlse_amd import NLSE, NLSEBatch
     from nlse_amd import workloads as W
 
-    sims, fields = W.config2(NLSE)
+    sims, fields = W.uneven_batch(NLSE)
     batch = NLSEBatch(sims)
     eng, buf, _ = batch.prepare(fields, 1e-5)
     P = eng.total
@@ -388,7 +388,7 @@ def bench_c2(args) -> int:
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic",
-        "config": {"workload": "config-2: 64 NLSE fields, N uniform on {256, 512, 1024} (rng 2), n2 = -10^U(-10,-8), "
+        "config": {"workload": "64 NLSE fields, N uniform on {256, 512, 1024} (rng 2), n2 = -10^U(-10,-8), "
                                "P ~ U(0.1, 2) W, Isat ~ U(1e4, 1e6), alpha=20, V=None, dz=1e-5, precision='single'",
                    "points": P, "batch": eng.config(), "graph_chunk": eng.graph_chunk},
         "roofline": {"bound": "hbm", "kernel": f"batch {dom} sweep", "achieved": achieved / 1e9,
@@ -404,7 +404,7 @@ def bench_c2(args) -> int:
         import oracle
 
         threads = min(os.cpu_count() or 1, 16)
-        osims, ofields = W.config2(oracle.NLSE, fft="pocketfft32", workers=threads, threads=threads)
+        osims, ofields = W.uneven_batch(oracle.NLSE, fft="pocketfft32", workers=threads, threads=threads)
         t0, n, pts = time.perf_counter(), 0, 0
         for s, E in zip(osims, ofields):
             A, A_sq = s._prepare_output_array(E, True)
@@ -417,22 +417,22 @@ def bench_c2(args) -> int:
         el = time.perf_counter() - t0
         result["cpu_baseline"] = {
             "value": pts / el / P, "unit": "batch-steps/s", "cores": threads, "kind": "port",
-            "sample": f"one split step of each of the first {n} config-2 fields on the CPU oracle (complex64, "
+            "sample": f"one split step of each of the first {n} batch fields on the CPU oracle (complex64, "
                       f"scipy.fft with {threads} workers), {pts} grid points in {el:.1f} s, scaled to the batch's "
                       f"{P} points; host CPU: {cpu_model()}"}
     print(json.dumps(result), flush=True)
     return 0
 
 
-def bench_c3(args) -> int:
+def bench_gvd3d(args) -> int:
     """3-D (x, y, t) NLSE 512x512x64 with GVD: four sweeps per step."""
     import torch
 
     from nlse_amd import NLSE_3d
     from nlse_amd import workloads as W
 
-    sim = W.config3(NLSE_3d)
-    E0 = W.config3_field(sim)
+    sim = W.gvd_3d(NLSE_3d)
+    E0 = W.sech_pulse_field(sim)
     sim._own_propagator()
     sim._send_arrays_to_gpu()
     A, _ = sim._prepare_output_array(E0, True)
@@ -463,7 +463,7 @@ def bench_c3(args) -> int:
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic",
-        "config": {"workload": "config-3: NLSE_3d 512x512x64, D0=-1e-26, window [4e-3, 4e-3, 2e-12], n2=-1e-9, "
+        "config": {"workload": "NLSE_3d 512x512x64, D0=-1e-26, window [4e-3, 4e-3, 2e-12], n2=-1e-9, "
                                "alpha=0, Isat=inf, dz=1e-5, Gaussian x sech(t/200fs), precision='single'",
                    "grid": list(A.shape)},
         "roofline": {"bound": "hbm", "kernel": "step (4 sweeps)", "achieved": sps * nbytes / 1e9,
@@ -476,7 +476,7 @@ def bench_c3(args) -> int:
     return 0
 
 
-def bench_c4(args) -> int:
+def bench_corrector(args) -> int:
     """Learned corrector: accuracy at 128^2 (trained there) against the FP64
     oracle over L = 10 mm, and throughput at 1024^2 of the corrected coarse
     loop (coarse step dz_c = 8 dz + network) against the fine-step loop."""
@@ -563,9 +563,9 @@ def bench_c4(args) -> int:
         "vs_baseline": None,
         "dtype": "c64 solver, bf16 MFMA network (fp32 accumulation)",
         "data": "synthetic",
-        "config": {"workload": "config-4: FNO corrector (4 spectral layers, 32 channels, 32x32 modes) of coarse "
+        "config": {"workload": "FNO corrector (4 spectral layers, 32 channels, 32x32 modes) of coarse "
                                "steps dz_c = 8e-5 vs fine dz = 1e-5; trained at 128^2 on 1600 pairs from the "
-                               "plain solver (config-0 material x U(0.5,1.5), speckle inputs); throughput at "
+                               "plain solver (README-example material x U(0.5,1.5), speckle inputs); throughput at "
                                "1024^2",
                    "train_seconds": t_train, "train_pairs": 1600},
         "fine_steps_per_s": 1.0 / t_fine,

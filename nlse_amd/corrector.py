@@ -8,7 +8,7 @@ learned corrector (nlse_amd.fno).
 
 Pairs: (X, Y, F) with X a field along a trajectory of the plain fine-step
 solver, Y one coarse split step (dz_c = R dz) of X, F the R fine steps of X.
-Material parameters are the config-0 values scaled by U(0.5, 1.5) per
+Material parameters are the README-example values scaled by U(0.5, 1.5) per
 trajectory (n2, power, Isat, alpha); inputs are Gaussian beams with a
 random-phase speckle (seeded).
 """
@@ -39,7 +39,7 @@ def _speckle_field(sim, rng) -> np.ndarray:
 
 
 def material(rng, scale: bool = True) -> dict:
-    """Config-0 material parameters, each scaled by U(0.5, 1.5)."""
+    """README-example material parameters, each scaled by U(0.5, 1.5)."""
     f = (lambda: rng.uniform(0.5, 1.5)) if scale else (lambda: 1.0)
     return dict(alpha=20 * f(), power=1.05 * f(), n2=-1.6e-9 * f(), Isat=1e5 * f())
 

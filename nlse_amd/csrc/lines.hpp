@@ -193,7 +193,7 @@ struct ColCfg {
   // with NLSE_COL_PREFER_PAIRS every thread owns two adjacent columns
   // (16-byte accesses) whenever the tile has two or more; measured slower
   // where the tile fits one column per thread (1024^2 column sweep 11.7 ->
-  // 13.3 us, 3-D config 3.76k -> 3.49k steps/s), so pairs are used only when
+  // 13.3 us, 3-D 512x512x64 workload 3.76k -> 3.49k steps/s), so pairs are used only when
   // the tile is wider than WTMAX threads
   static constexpr int WTP = (NLSE_COL_PREFER_PAIRS && W >= 2) ? W / 2 : W;
   static constexpr int WT = WTP < WTMAX ? WTP : WTMAX;

@@ -1,19 +1,22 @@
-"""Synthetic inputs of the benchmark configurations (frozen recipes).
+"""Synthetic inputs of the benchmark workloads (frozen recipes).
 
 Each builder takes the solver class (``nlse_amd.NLSE`` or an oracle class)
 so the engine and the CPU oracle are set up from the same numbers.
 
-  config0  2-D NLSE, README example form: 512^2, alpha=20, P=1.05 W,
-           window 4 w (w = 2.23 mm), n2=-1.6e-9, Isat=1e5, L=10 mm,
-           dz=1e-5 (1001 steps by the reference loop rule), Gaussian beam
-           times (1 + 0.01 complex Gaussian noise, seed 0), potential
-           V = -1e-4 exp(-r^2/(70 um)^2).
-  config1  the same physics at 2048^2, dz=5e-6 (2001 steps), nonlocal
-           length 20 um (optional), Gaussian beam plus a random-phase
-           speckle low-passed to 1/8 of the band (seed 1).
-  config2  64 uneven instances (256^2, 512^2, 1024^2) with random n2,
-           power and Isat, 500 steps each.
-  config3  3-D NLSE 512x512x64 with GVD (D0 = -1e-26), sech pulse.
+  readme_nlse            2-D NLSE, README example form: 512^2, alpha=20,
+                         P=1.05 W, window 4 w (w = 2.23 mm), n2=-1.6e-9,
+                         Isat=1e5, L=10 mm, dz=1e-5 (1001 steps by the
+                         reference loop rule), Gaussian beam times (1 + 0.01
+                         complex Gaussian noise, seed 0), potential
+                         V = -1e-4 exp(-r^2/(70 um)^2).
+  speckle_nonlocal_nlse  the same physics at 2048^2, dz=5e-6 (2001 steps),
+                         nonlocal length 20 um (optional), Gaussian beam
+                         plus a random-phase speckle low-passed to 1/8 of
+                         the band (seed 1).
+  uneven_batch           64 uneven instances (256^2, 512^2, 1024^2) with
+                         random n2, power and Isat, 500 steps each.
+  gvd_3d                 3-D NLSE 512x512x64 with GVD (D0 = -1e-26), sech
+                         pulse.
 """
 
 from __future__ import annotations
@@ -35,7 +38,7 @@ def _potential(sim):
     return (-1e-4 * np.exp(-(sim.XX**2 + sim.YY**2) / (70e-6) ** 2)).astype(np.float32)
 
 
-def config0(cls, N: int = 512, **kw):
+def readme_nlse(cls, N: int = 512, **kw):
     sim = _common(cls, N, **kw)
     sim.delta_z = 1e-5
     sim.V = _potential(sim)
@@ -46,7 +49,7 @@ def gaussian(sim):
     return np.exp(-(sim.XX.astype(np.float64) ** 2 + sim.YY.astype(np.float64) ** 2) / WAIST**2)
 
 
-def config0_field(sim, seed: int = 0) -> np.ndarray:
+def readme_field(sim, seed: int = 0) -> np.ndarray:
     rng = np.random.default_rng(seed)
     G = gaussian(sim)
     nr = rng.standard_normal(G.shape)
@@ -54,14 +57,14 @@ def config0_field(sim, seed: int = 0) -> np.ndarray:
     return (G * (1 + 0.01 * (nr + 1j * ni))).astype(np.complex64)
 
 
-def config1(cls, N: int = 2048, nonlocal_length: float = 20e-6, **kw):
+def speckle_nonlocal_nlse(cls, N: int = 2048, nonlocal_length: float = 20e-6, **kw):
     sim = _common(cls, N, nl_length=nonlocal_length, **kw)
     sim.delta_z = 5e-6
     sim.V = _potential(sim)
     return sim
 
 
-def config1_field(sim, seed: int = 1) -> np.ndarray:
+def speckle_field(sim, seed: int = 1) -> np.ndarray:
     rng = np.random.default_rng(seed)
     ny, nx = sim.XX.shape
     phi = rng.uniform(0.0, 2 * np.pi, (ny, nx))
@@ -74,10 +77,10 @@ def config1_field(sim, seed: int = 1) -> np.ndarray:
     return (gaussian(sim) * (1 + S)).astype(np.complex64)
 
 
-def config2(cls, count: int = 64, seed: int = 2, sizes=(256, 512, 1024), **kw):
+def uneven_batch(cls, count: int = 64, seed: int = 2, sizes=(256, 512, 1024), **kw):
     """Uneven batch: N_i uniform on `sizes`, n2_i = -10^U(-10,-8),
     P_i ~ U(0.1, 2) W, Isat_i ~ U(1e4, 1e6) W/m^2 (rng(seed), drawn in that
-    order per instance), config-0 material otherwise, V = None,
+    order per instance), README-example material otherwise, V = None,
     dz = 1e-5; E0_i = Gaussian x (1 + 0.01 complex noise from rng(1000+i))."""
     rng = np.random.default_rng(seed)
     sims, fields = [], []
@@ -96,10 +99,10 @@ def config2(cls, count: int = 64, seed: int = 2, sizes=(256, 512, 1024), **kw):
     return sims, fields
 
 
-def config3(cls, NXY: int = 512, NT: int = 64, **kw):
+def gvd_3d(cls, NXY: int = 512, NT: int = 64, **kw):
     """3-D (x, y, t): NLSE_3d, window [4e-3, 4e-3, 2e-12], D0 = -1e-26,
     n2 = -1e-9, alpha = 0, Isat = inf, dz = 1e-5; the energy is chosen so
-    the peak power is 1 W for the sech pulse of config3_field."""
+    the peak power is 1 W for the sech pulse of sech_pulse_field."""
     T0 = 200e-15
     t, dT = np.linspace(-1e-12, 1e-12, NT, retstep=True)
     energy = float(np.sum(1.0 / np.cosh(t / T0) ** 2) * dT)
@@ -109,7 +112,7 @@ def config3(cls, NXY: int = 512, NT: int = 64, **kw):
     return sim
 
 
-def config3_field(sim) -> np.ndarray:
+def sech_pulse_field(sim) -> np.ndarray:
     """Gaussian in space (waist 1 mm) times sech(t / 200 fs), layout (NY, NX, NT)."""
     XX = sim.XX.astype(np.float64)
     YY = sim.YY.astype(np.float64)

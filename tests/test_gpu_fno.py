@@ -116,7 +116,7 @@ def _loop_errors(net, seed, nfine=64):
 
 
 def test_training_gain_and_fallback(gpu):
-    """Trained on 1600 pairs of the plain solver (config-0 material x
+    """Trained on 1600 pairs of the plain solver (README-example material x
     U(0.5,1.5), speckle inputs), the corrected coarse step is closer to the
     8 fine steps than the plain coarse step on held-out trajectories, and
     the corrected loop beats the coarse loop against the fine-step solver

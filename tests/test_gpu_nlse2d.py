@@ -20,7 +20,7 @@ waist = 2.23e-3
 window = 4 * waist
 
 
-def config0(cls, N, mode=None, **extra):
+def readme_nlse(cls, N, mode=None, **extra):
     kw = dict(alpha=20, power=1.05, window=window, n2=-1.6e-9, V=None, L=10e-3, NX=N, NY=N, Isat=1e5,
               wvl=780e-9)
     kw.update(extra)
@@ -33,7 +33,7 @@ def config0(cls, N, mode=None, **extra):
     return s
 
 
-def config0_field(s, seed=0):
+def readme_field(s, seed=0):
     rng = np.random.default_rng(seed)
     G = np.exp(-(s.XX.astype(np.float64) ** 2 + s.YY.astype(np.float64) ** 2) / waist**2)
     nr = rng.standard_normal(G.shape)
@@ -46,12 +46,12 @@ def config0_field(s, seed=0):
 def test_out_field_matches_oracle(gpu, N, z, precision):
     from nlse_amd import NLSE
 
-    sim = config0(NLSE, N)
-    E0 = config0_field(sim)
+    sim = readme_nlse(NLSE, N)
+    E0 = readme_field(sim)
     out = sim.out_field(E0, z, precision=precision, verbose=False)
-    ref = config0(oracle.NLSE, N, mode="c64").out_field(E0, z, precision=precision)
+    ref = readme_nlse(oracle.NLSE, N, mode="c64").out_field(E0, z, precision=precision)
     err = rel_l2(out, ref)
-    err64 = rel_l2(out, config0(oracle.NLSE, N, mode="c128").out_field(E0, z, precision=precision))
+    err64 = rel_l2(out, readme_nlse(oracle.NLSE, N, mode="c128").out_field(E0, z, precision=precision))
     print(f"N={N} steps={oracle.count_steps(z, 1e-5)} {precision}: vs c64 oracle {err:.2e}, vs FP64 {err64:.2e}")
     assert out.shape == E0.shape and out.dtype == np.complex64
     assert err < PARITY
@@ -62,15 +62,15 @@ def test_loss_semantics_switch(gpu):
     from nlse_amd import NLSE
 
     N, z = 128, 2e-4
-    sim = config0(NLSE, N)
+    sim = readme_nlse(NLSE, N)
     sim.loss_saturates = True
-    E0 = config0_field(sim)
+    E0 = readme_field(sim)
     out = sim.out_field(E0, z, verbose=False)
-    ref_o = config0(oracle.NLSE, N, mode="c64")
+    ref_o = readme_nlse(oracle.NLSE, N, mode="c64")
     ref_o.loss_saturates = True
     ref = ref_o.out_field(E0, z)
     assert rel_l2(out, ref) < PARITY
-    cpu_sem = config0(oracle.NLSE, N, mode="c64").out_field(E0, z)
+    cpu_sem = readme_nlse(oracle.NLSE, N, mode="c64").out_field(E0, z)
     assert rel_l2(out, cpu_sem) > 10 * PARITY  # the two semantics do differ
 
 
@@ -84,7 +84,7 @@ def test_broadcasting_parameters(gpu):
     alpha = np.array([20.0, 10.0, 0.0]).reshape(B, 1, 1)
     sim = NLSE(alpha, 1.05, window, n2, None, 10e-3, NX=N, NY=N, Isat=1e5)
     sim.delta_z = 1e-5
-    E0 = config0_field(sim)
+    E0 = readme_field(sim)
     Eb = np.stack([E0] * B)
     out = sim.out_field(Eb, z, verbose=False)
     assert out.shape == Eb.shape
@@ -184,7 +184,7 @@ def test_callbacks_and_step_rule(gpu):
 
 @pytest.mark.parametrize("precision", ["single", "double"])
 def test_full_length_parity(gpu, precision):
-    """Config-0 physics over the full L = 10 mm (1001 steps) at 128^2: the
+    """README-example physics over the full L = 10 mm (1001 steps) at 128^2: the
     accumulated error stays within the parity bound against the complex64
     oracle with a correctly rounded FFT; the distance to the FP64 oracle and
     to scipy's own single-precision FFT (whose gain bias accumulates) is
@@ -192,12 +192,12 @@ def test_full_length_parity(gpu, precision):
     from nlse_amd import NLSE
 
     N, z = 128, 10e-3
-    sim = config0(NLSE, N)
-    E0 = config0_field(sim)
+    sim = readme_nlse(NLSE, N)
+    E0 = readme_field(sim)
     out = sim.out_field(E0, z, precision=precision, verbose=False)
-    ref = config0(oracle.NLSE, N, mode="c64").out_field(E0, z, precision=precision)
-    ref64 = config0(oracle.NLSE, N, mode="c128").out_field(E0, z, precision=precision)
-    pocket = config0(oracle.NLSE, N, mode="pocket").out_field(E0, z, precision=precision)
+    ref = readme_nlse(oracle.NLSE, N, mode="c64").out_field(E0, z, precision=precision)
+    ref64 = readme_nlse(oracle.NLSE, N, mode="c128").out_field(E0, z, precision=precision)
+    pocket = readme_nlse(oracle.NLSE, N, mode="pocket").out_field(E0, z, precision=precision)
     err, err64, errp = rel_l2(out, ref), rel_l2(out, ref64), rel_l2(out, pocket)
     print(f"1001 steps {precision}: vs c64 oracle {err:.2e}, vs FP64 oracle {err64:.2e} "
           f"(c64 oracle vs FP64 {rel_l2(ref, ref64):.2e}), vs pocketfft32 c64 {errp:.2e}")
@@ -205,16 +205,16 @@ def test_full_length_parity(gpu, precision):
 
 
 def test_full_length_parity_1024(gpu):
-    """Config-0 physics over the full L = 10 mm (1001 steps) at 1024^2, where
+    """README-example physics over the full L = 10 mm (1001 steps) at 1024^2, where
     the row sweep runs with 8 points per thread: the accumulated error stays
     within the parity bound against the complex64 oracle."""
     from nlse_amd import NLSE
 
     N, z = 1024, 10e-3
-    sim = config0(NLSE, N)
-    E0 = config0_field(sim)
+    sim = readme_nlse(NLSE, N)
+    E0 = readme_field(sim)
     out = sim.out_field(E0, z, verbose=False)
-    ref = config0(oracle.NLSE, N, mode="c64").out_field(E0, z)
+    ref = readme_nlse(oracle.NLSE, N, mode="c64").out_field(E0, z)
     err = rel_l2(out, ref)
     print(f"1001 steps at 1024^2: vs c64 oracle {err:.2e}")
     assert err < PARITY

@@ -299,7 +299,7 @@ def test_nlse_long_line_parity(gpu, N, nl, precision):
 @pytest.mark.parametrize("ny,nx", [(128, 128), (64, 128)])
 def test_nlse_lorentzian_matches_oracle(gpu, precision, ny, nx):
     """nonlocal_mode = "lorentzian": the response applied in Fourier space as
-    1/(1 + k^2 l^2) (config-1 text; no reference counterpart), one
+    1/(1 + k^2 l^2) (no reference counterpart), one
     component on the full torus."""
     g, o = _pair("NLSE", ny, nx, 3.5)
     for s in (g, o):
@@ -334,19 +334,19 @@ def test_cnlse_lorentzian_matches_oracle(gpu):
 @pytest.mark.xfail(strict=False, reason="measured 1.01e-5: over 2001 steps the FP32 transform rounding noise "
                                         "(growing as sqrt(steps), 2.4e-7 per step) reaches the 1e-5 bound against "
                                         "the oracle's exactly rounded transforms")
-def test_config1_physics_full_length_1024(gpu):
-    """Config-1 physics (speckle input, real V, nonlocal K0 response of
+def test_speckle_nonlocal_full_length_1024(gpu):
+    """Speckle-workload physics (speckle input, real V, nonlocal K0 response of
     nl_length 20 um, dz = 5e-6) over the full L = 10 mm (2001 steps) at
     1024^2 against the complex64 oracle (measured 1.01e-5; 512^2: 1.08e-5;
     both complex64 paths are 2.7-3.2e-5 from the FP64 oracle)."""
     from nlse_amd import NLSE
     from nlse_amd import workloads as W
 
-    g = W.config1(NLSE, 1024)
-    o = W.config1(oracle.NLSE, 1024, mode="c64", workers=16)
-    E0 = W.config1_field(g)
+    g = W.speckle_nonlocal_nlse(NLSE, 1024)
+    o = W.speckle_nonlocal_nlse(oracle.NLSE, 1024, mode="c64", workers=16)
+    E0 = W.speckle_field(g)
     out = g.out_field(E0, 10e-3, verbose=False)
     ref = o.out_field(E0, 10e-3)
     err = rel_l2(out, ref)
-    print(f"config-1 physics, 2001 steps at 1024^2 (kernel {o.nl_profile.shape}): vs c64 oracle {err:.2e}")
+    print(f"speckle-workload physics, 2001 steps at 1024^2 (kernel {o.nl_profile.shape}): vs c64 oracle {err:.2e}")
     assert err < PARITY
