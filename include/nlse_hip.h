@@ -118,6 +118,12 @@ int nlse_cavity_drive(void* A_dev, int64_t n_items, int64_t item_size, int64_t i
  * (power integrals of nlse.py:297-303 and callbacks.py:54). */
 int nlse_sumsq(const void* A_dev, int64_t n_items, int64_t item_size, double* out_dev, void* stream);
 
+/* Per-item FP64 sum of |A - B|^2 (differences formed in FP64) -> out_dev[n_items].
+ * No counterpart in the reference: the error measure and the drift test of
+ * the learned corrector (relative L2 of two complex64 fields). */
+int nlse_sumsq_diff(const void* A_dev, const void* B_dev, int64_t n_items, int64_t item_size, double* out_dev,
+                    void* stream);
+
 /* Nonlinear index change of callbacks.evaluate_delta_n (callbacks.py:57-82)
  * and callbacks.adapt_delta_z (callbacks.py:85-112) in FP32, rounded op by
  * op in the reference's evaluation order:

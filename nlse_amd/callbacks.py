@@ -26,7 +26,7 @@ def norm(simu, A, z: float, i: int, save_every: int, norms) -> None:
     """Save sum |A|^2 over all axes every save_every steps (callbacks.py:32-54;
     no dx*dy factor, like the reference)."""
     if i % save_every == 0:
-        s = kernels_hip.sumsq(A.reshape(-1, A.shape[-1]), 1).sum()
+        s = kernels_hip.sumsq(A.reshape(-1, A.shape[-1]), 1).cpu().numpy().sum()
         norms[i // save_every] = float(s)
 
 

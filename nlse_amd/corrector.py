@@ -20,6 +20,7 @@ import time
 import numpy as np
 import torch
 
+from . import kernels_hip
 from .fno import FNO
 from .nlse import NLSE, count_steps
 
@@ -84,10 +85,11 @@ def make_pairs(N: int = 128, trajectories: int = 16, coarse_steps: int = 24, rat
 
 
 def residual_scale(pairs: dict) -> float:
-    """RMS of the relative residual |F - Y| / |Y| over the pairs."""
-    return float(torch.sqrt(torch.mean(rel_err(pairs["Y"], pairs["F"]) ** 2 *
-                                       (torch.view_as_real(pairs["F"]).double().pow(2).flatten(1).sum(1) /
-                                        torch.view_as_real(pairs["Y"]).double().pow(2).flatten(1).sum(1)))))
+    """RMS of the relative residual |F - Y| / |Y| over the pairs (FP64 sums
+    on the device, the few per-pair ratios on the host)."""
+    d = kernels_hip.sumsq_diff(pairs["F"], pairs["Y"], 2).cpu().numpy()
+    y = kernels_hip.sumsq(pairs["Y"], 2).cpu().numpy()
+    return float(np.sqrt(np.mean(d / y)))
 
 
 def train(net: FNO, pairs: dict, steps: int = 1000, batch: int = 8, lr: float = 1e-3, seed: int = 0,
@@ -111,11 +113,12 @@ def train(net: FNO, pairs: dict, steps: int = 1000, batch: int = 8, lr: float = 
     return hist
 
 
-def rel_err(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """Per-sample relative L2 error |a - b| / |b| (FP64)."""
-    d = torch.view_as_real(a - b).double().pow(2).flatten(1).sum(1)
-    r = torch.view_as_real(b).double().pow(2).flatten(1).sum(1)
-    return torch.sqrt(d / r)
+def rel_err(a: torch.Tensor, b: torch.Tensor) -> np.ndarray:
+    """Per-sample relative L2 error |a - b| / |b| of [B][H][W] fields (FP64
+    sums on the device)."""
+    d = kernels_hip.sumsq_diff(a.contiguous(), b.contiguous(), 2).cpu().numpy()
+    r = kernels_hip.sumsq(b.contiguous(), 2).cpu().numpy()
+    return np.sqrt(d / r)
 
 
 def evaluate(net: FNO, pairs: dict, batch: int = 8) -> dict:
@@ -130,8 +133,8 @@ def evaluate(net: FNO, pairs: dict, batch: int = 8) -> dict:
         A = net.correct(Y, X)
         e_c.append(rel_err(Y, F))
         e_n.append(rel_err(A, F))
-    e_c = torch.cat(e_c)
-    e_n = torch.cat(e_n)
+    e_c = np.concatenate(e_c)
+    e_n = np.concatenate(e_n)
     return {"coarse": float(e_c.mean()), "corrected": float(e_n.mean()),
             "gain": float(e_c.mean() / e_n.mean())}
 
@@ -165,9 +168,8 @@ class CorrectedSolver:
             Y.copy_(A)
             ec.run(Y, 1)
             self.net.correct(Y[None], A[None], out=B[None])
-            p_y = torch.view_as_real(Y).double().pow(2).sum()
-            p_b = torch.view_as_real(B).double().pow(2).sum()
-            if float(torch.abs(p_b - p_y) / p_y) > self.drift_tol:
+            p = torch.cat((kernels_hip.sumsq(Y[None], 2), kernels_hip.sumsq(B[None], 2))).cpu().numpy()
+            if abs(p[1] - p[0]) / p[0] > self.drift_tol:
                 ef.run(A, self.ratio)
                 stats["fallbacks"] += 1
             else:

@@ -268,15 +268,25 @@ __global__ void k_drive(float2* A, int64_t items, int64_t isz, int64_t str, cons
 constexpr int SUMSQ_THREADS = 256;
 constexpr int SUMSQ_CHUNK = SUMSQ_THREADS * 16;
 
-__global__ __launch_bounds__(SUMSQ_THREADS) void k_sumsq_part(const float2* A, int64_t isz, int64_t nchunks,
-                                                             double* part) {
+// DIFF: the sum of |A - B|^2 (the difference formed in FP64, exact for
+// two float32 values of similar magnitude).
+template <bool DIFF>
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_sumsq_part(const float2* A, const float2* B, int64_t isz,
+                                                             int64_t nchunks, double* part) {
   const int64_t item = blockIdx.x / nchunks, ch = blockIdx.x % nchunks;
   const float2* a = A + item * isz;
+  const float2* b = DIFF ? B + item * isz : nullptr;
   const int64_t beg = ch * SUMSQ_CHUNK, end = beg + SUMSQ_CHUNK < isz ? beg + SUMSQ_CHUNK : isz;
   double acc = 0.0;
   for (int64_t i = beg + threadIdx.x; i < end; i += SUMSQ_THREADS) {
     const float2 v = a[i];
-    acc += (double)v.x * v.x + (double)v.y * v.y;
+    double x = v.x, y = v.y;
+    if constexpr (DIFF) {
+      const float2 w = b[i];
+      x -= (double)w.x;
+      y -= (double)w.y;
+    }
+    acc += x * x + y * y;
   }
   __shared__ double red[SUMSQ_THREADS];
   red[threadIdx.x] = acc;
@@ -471,19 +481,32 @@ int nlse_cavity_drive(void* A, int64_t items, int64_t isz, int64_t str, const vo
   return launch_status("k_drive");
 }
 
-int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* stream) {
-  if (!A || !out || items < 1 || isz < 1) return fail(NLSE_ERR_ARG, "bad sumsq arguments");
+static int sumsq_launch(const void* A, const void* B, int64_t items, int64_t isz, double* out, void* stream) {
   const int64_t nchunks = (isz + SUMSQ_CHUNK - 1) / SUMSQ_CHUNK;
   double* part = nullptr;
   int rc = scratch_doubles((size_t)(items * nchunks), &part);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_sumsq_part, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s, (const float2*)A,
-                     isz, nchunks, part);
+  if (B)
+    hipLaunchKernelGGL(k_sumsq_part<true>, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s,
+                       (const float2*)A, (const float2*)B, isz, nchunks, part);
+  else
+    hipLaunchKernelGGL(k_sumsq_part<false>, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s,
+                       (const float2*)A, (const float2*)nullptr, isz, nchunks, part);
   rc = launch_status("k_sumsq_part");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sumsq_final, dim3((unsigned)items), dim3(SUMSQ_THREADS), 0, s, part, nchunks, out);
   return launch_status("k_sumsq_final");
+}
+
+int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* stream) {
+  if (!A || !out || items < 1 || isz < 1) return fail(NLSE_ERR_ARG, "bad sumsq arguments");
+  return sumsq_launch(A, nullptr, items, isz, out, stream);
+}
+
+int nlse_sumsq_diff(const void* A, const void* B, int64_t items, int64_t isz, double* out, void* stream) {
+  if (!A || !B || !out || items < 1 || isz < 1) return fail(NLSE_ERR_ARG, "bad sumsq_diff arguments");
+  return sumsq_launch(A, B, items, isz, out, stream);
 }
 
 int nlse_delta_n(const void* A, int64_t items, int64_t isz, float qc, float qe, nlse_bparam coef,

@@ -258,30 +258,36 @@ class FNO:
                                           D.stream_ptr()), "fno_adam")
 
     # ---------------------------------------------------- field interface
-    def _scales(self, Y: torch.Tensor):
+    def _scales(self, Y: torch.Tensor) -> np.ndarray:
+        """Per-sample rms s = sqrt(sum |Y|^2 / (H W)) (FP64 sum on the
+        device, 1 where Y is zero), as host float32."""
         from . import kernels_hip
 
-        B = Y.shape[0]
-        ss = kernels_hip.sumsq(Y, 2).reshape(B)
-        s = torch.sqrt(ss / (self.H * self.W)).to(torch.float32)
-        s = torch.where(s > 0, s, torch.ones_like(s))
-        return s.contiguous(), (1.0 / s).contiguous()
+        ss = kernels_hip.sumsq(Y, 2).reshape(Y.shape[0]).cpu().numpy()
+        s = np.sqrt(ss / (self.H * self.W)).astype(np.float32)
+        return np.where(s > 0, s, np.float32(1.0)).astype(np.float32)
+
+    @staticmethod
+    def _dev(v: np.ndarray) -> torch.Tensor:
+        return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float32)).to(D.device())
 
     def features(self, Y: torch.Tensor, X: torch.Tensor):
-        """Network input and scales for complex64 fields Y, X [B][H][W]."""
+        """Network input and per-sample scales (host) for complex64 fields
+        Y, X [B][H][W]."""
         B = Y.shape[0]
-        s, inv = self._scales(Y)
+        s = self._scales(Y)
+        inv = self._dev(np.float32(1.0) / s)
         x_in = self._buf("x_in", (B, 4, self.H, self.W))
         _lib.check(self.lib.nlse_fno_pack(Y.data_ptr(), X.data_ptr(), inv.data_ptr(), x_in.data_ptr(), B,
                                           self.H * self.W, D.stream_ptr()), "fno_pack")
-        return x_in, s, inv
+        return x_in, s
 
     def correct(self, Y: torch.Tensor, X: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Corrected fields Y + s * FNO(Y/s, X/s)."""
-        x_in, s, _ = self.features(Y, X)
+        x_in, s = self.features(Y, X)
         o = self.forward(x_in)
         A = torch.empty_like(Y) if out is None else out
-        s2 = (s * self.res_scale).contiguous()
+        s2 = self._dev(s * np.float32(self.res_scale))
         _lib.check(self.lib.nlse_fno_apply(Y.data_ptr(), o.data_ptr(), s2.data_ptr(), A.data_ptr(), Y.shape[0],
                                            self.H * self.W, D.stream_ptr()), "fno_apply")
         return A
@@ -290,9 +296,9 @@ class FNO:
         """One Adam step on the batch (Y, X) -> F; returns the loss (MSE of
         the scaled residual)."""
         B = Y.shape[0]
-        x_in, s, inv = self.features(Y, X)
+        x_in, s = self.features(Y, X)
         tgt = self._buf("tgt", (B, 2, self.H, self.W))
-        inv2 = (inv / self.res_scale).contiguous()
+        inv2 = self._dev(np.float32(1.0) / s / np.float32(self.res_scale))
         _lib.check(self.lib.nlse_fno_residual(F.data_ptr(), Y.data_ptr(), inv2.data_ptr(), tgt.data_ptr(), B,
                                               self.H * self.W, D.stream_ptr()), "fno_residual")
         out = self.forward(x_in, keep=True)
@@ -303,4 +309,4 @@ class FNO:
                                          part.data_ptr(), nparts, D.stream_ptr()), "fno_mse")
         self.backward(dout)
         self.adam(lr)
-        return float(part.sum()) / out.numel()
+        return float(part.cpu().numpy().sum()) / out.numel()

@@ -181,6 +181,19 @@ def sumsq(A: torch.Tensor, tdims: int) -> torch.Tensor:
     return out.reshape(A.shape[:A.dim() - tdims])
 
 
+def sumsq_diff(A: torch.Tensor, B: torch.Tensor, tdims: int) -> torch.Tensor:
+    """Per-item FP64 sum of |A - B|^2 over the last ``tdims`` axes."""
+    _check_field(A)
+    _check_field(B)
+    if tuple(A.shape) != tuple(B.shape):
+        raise ValueError(f"shapes differ: {tuple(A.shape)} and {tuple(B.shape)}")
+    items, isz = _items(A, tdims)
+    out = torch.empty(items, dtype=torch.float64, device=A.device)
+    _lib.check(_lib.load().nlse_sumsq_diff(A.data_ptr(), B.data_ptr(), items, isz, out.data_ptr(), D.stream_ptr()),
+               "sumsq_diff")
+    return out.reshape(A.shape[:A.dim() - tdims])
+
+
 # ------------------------------------------------ callbacks and helpers
 _NUM_KINDS = {torch.float32: 0, torch.float64: 1, torch.int32: 2, torch.int64: 3}
 

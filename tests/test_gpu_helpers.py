@@ -192,3 +192,17 @@ def test_propagator_table_equals_numpy(gpu, ny, nx, dz):
     want = np.exp(-1j * 0.5 * (s.Kxx**2 + s.Kyy**2) / s.k * s.delta_z).astype(np.complex64)
     assert got.dtype == np.complex64 and got.shape == (ny, nx)
     assert np.array_equal(got, want), np.count_nonzero(got != want)
+
+
+def test_sumsq_diff_matches_fp64(gpu):
+    """nlse_sumsq_diff: per-item FP64 sum of |A - B|^2 (the corrector's error
+    and drift measure) against numpy in FP64; zero for equal fields."""
+    from nlse_amd import kernels_hip as KH
+
+    a = _field((3, 50, 70), 11, scale=1.0)
+    b = (a + 1e-3 * _field((3, 50, 70), 12, scale=1.0)).astype(np.complex64)
+    got = KH.sumsq_diff(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), 2).cpu().numpy()
+    want = (np.abs(a.astype(np.complex128) - b.astype(np.complex128)) ** 2).sum(axis=(1, 2))
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+    A = torch.from_numpy(a).cuda()
+    assert np.all(KH.sumsq_diff(A, A.clone(), 2).cpu().numpy() == 0)
