@@ -124,6 +124,23 @@ int nlse_sumsq(const void* A_dev, int64_t n_items, int64_t item_size, double* ou
 int nlse_sumsq_diff(const void* A_dev, const void* B_dev, int64_t n_items, int64_t item_size, double* out_dev,
                     void* stream);
 
+/* Hamiltonian terms of the single-component 2-D model (no counterpart in
+ * the reference; the drift test of the learned corrector).  For
+ *   dA/dz = (i/2k) lap A + i (g I/(1 + I/Is) + vscale V) A,  I = |A|^2,
+ * H = dx dy sum[(1/2k)|grad A|^2 - G(I) - vscale Re(V) I] is conserved
+ * without loss, G(I) = g Is (I - Is ln(1 + I/Is)) (g I^2/2 for Is = inf).
+ * nlse_sumsq_weighted: per item sum of (wx[x] + wy[y]) |A[y][x]|^2 over an
+ * (ny, nx) item, FP64 (wx = kx^2, wy = ky^2 on the transformed field gives
+ * N sum |grad A|^2 by Parseval).  nlse_nl_energy: per item sum of
+ * G(|A|^2) + vscale Re(V) |A|^2, FP64, inv_isat = 1/Is (0: Kerr); V as for
+ * nlse_plan_set_potential (v_kind 0 none, 1 float32, 2 complex64; stride 0
+ * shares one plane). */
+int nlse_sumsq_weighted(const void* A_dev, int64_t n_items, int64_t ny, int64_t nx, const double* wx_dev,
+                        const double* wy_dev, double* out_dev, void* stream);
+int nlse_nl_energy(const void* A_dev, int64_t n_items, int64_t item_size, double g, double inv_isat,
+                   const void* V_dev, int v_kind, int64_t v_item_stride, double vscale, double* out_dev,
+                   void* stream);
+
 /* Nonlinear index change of callbacks.evaluate_delta_n (callbacks.py:57-82)
  * and callbacks.adapt_delta_z (callbacks.py:85-112) in FP32, rounded op by
  * op in the reference's evaluation order:

@@ -74,6 +74,9 @@ SIGNATURES = {
                                   c_void_p]),
     "nlse_sumsq": (c_int, [c_void_p, c_i64, c_i64, c_void_p, c_void_p]),
     "nlse_sumsq_diff": (c_int, [c_void_p, c_void_p, c_i64, c_i64, c_void_p, c_void_p]),
+    "nlse_sumsq_weighted": (c_int, [c_void_p, c_i64, c_i64, c_i64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nlse_nl_energy": (c_int, [c_void_p, c_i64, c_i64, c_double, c_double, c_void_p, c_int, c_i64, c_double,
+                               c_void_p, c_void_p]),
     "nlse_delta_n": (c_int, [c_void_p, c_i64, c_i64, C.c_float, C.c_float, BParam, BParam, c_void_p, c_void_p]),
     "nlse_delta_n_max": (c_int, [c_void_p, c_i64, c_i64, C.c_float, C.c_float, BParam, BParam, c_void_p,
                                  c_void_p]),

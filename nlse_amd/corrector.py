@@ -20,6 +20,7 @@ import time
 import numpy as np
 import torch
 
+from . import _device as D
 from . import kernels_hip
 from .fno import FNO
 from .nlse import NLSE, count_steps
@@ -139,13 +140,54 @@ def evaluate(net: FNO, pairs: dict, batch: int = 8) -> dict:
             "gain": float(e_c.mean() / e_n.mean())}
 
 
+def hamiltonian(sim: NLSE, A: torch.Tensor, plan=None) -> np.ndarray:
+    """H = T - U per realisation (see hamiltonian_terms)."""
+    T, U = hamiltonian_terms(sim, A, plan)
+    return T - U
+
+
+def hamiltonian_terms(sim: NLSE, A: torch.Tensor, plan=None):
+    """(T, U) with H = T - U, T = dx dy sum (1/2k)|grad A|^2 and
+    U = dx dy sum[G(|A|^2) + V'|A|^2]: H is, per realisation of
+    a single-component 2-D field [..., NY, NX] (device), the quantity the
+    split step conserves without loss: the linear step is exp(-i k_perp^2
+    dz/2k) (nlse.py:188-197) and the nonlinear phase rate g I/(1 + I/Is) + V'
+    with g = k/2 n2 c eps0, Is = 2 I_sat/(eps0 c), V' = k/2 Re V
+    (nlse.py:410-487); G is the antiderivative of the phase rate.  The
+    gradient term comes from the transformed field by Parseval (FP32
+    transform, FP64 sums); scalar n2 and I_sat only."""
+    from scipy.constants import c, epsilon_0
+
+    from .fft import HipFFTPlan
+
+    n2 = np.asarray(D.to_host(sim.n2), dtype=np.float64)
+    isat = np.asarray(D.to_host(sim.I_sat), dtype=np.float64)
+    if n2.size != 1 or isat.size != 1:
+        raise ValueError("hamiltonian takes scalar n2 and I_sat")
+    g = sim.k / 2 * float(n2.reshape(-1)[0]) * c * epsilon_0
+    isat_f = 2 * float(isat.reshape(-1)[0]) / (epsilon_0 * c)
+    inv_isat = 0.0 if not np.isfinite(isat_f) else 1.0 / isat_f
+    Ah = A.clone()
+    plan = plan if plan is not None else HipFFTPlan(tuple(A.shape), np.complex64, ndim=2)
+    plan.fft(Ah, Ah)
+    kin = kernels_hip.sumsq_weighted(Ah, np.asarray(sim.Kx, np.float64) ** 2,
+                                     np.asarray(sim.Ky, np.float64) ** 2).cpu().numpy()
+    V = None if sim.V is None else sim.V
+    pot = kernels_hip.nl_energy(A, g, inv_isat, V, sim.k / 2).cpu().numpy()
+    npts = A.shape[-1] * A.shape[-2]
+    cell = sim.delta_X * sim.delta_Y
+    return cell * kin / npts / (2 * sim.k), cell * pot
+
+
 class CorrectedSolver:
     """Coarse steps of ratio * delta_z corrected by the network, with the
     plain fine steps as fallback when the corrected step changes the power
-    by more than `drift_tol` (relative) from the coarse step's power."""
+    by more than `drift_tol` (relative) from the coarse step's power, or,
+    with `h_tol` set, its Hamiltonian by more than `h_tol` relative to
+    |T| + |U| of the coarse step (hamiltonian_terms)."""
 
-    def __init__(self, sim: NLSE, net: FNO, ratio: int = 8, drift_tol: float = 1e-4):
-        self.sim, self.net, self.ratio, self.drift_tol = sim, net, ratio, drift_tol
+    def __init__(self, sim: NLSE, net: FNO, ratio: int = 8, drift_tol: float = 1e-4, h_tol: float | None = 1e-3):
+        self.sim, self.net, self.ratio, self.drift_tol, self.h_tol = sim, net, ratio, drift_tol, h_tol
 
     def out_field(self, E0: np.ndarray, z: float):
         import copy
@@ -159,9 +201,15 @@ class CorrectedSolver:
         coarse.delta_z = sim.delta_z * self.ratio
         A, ef = _prepared(sim, E0)
         _, ec = _prepared(coarse, E0)
-        stats = {"coarse_steps": fine // self.ratio, "fallbacks": 0, "fine_steps": fine % self.ratio}
+        stats = {"coarse_steps": fine // self.ratio, "fallbacks": 0, "fallbacks_power": 0, "fallbacks_h": 0,
+                 "fine_steps": fine % self.ratio, "max_power_drift": 0.0, "max_h_drift": 0.0}
         Y = torch.empty_like(A)
         B = torch.empty_like(A)
+        plan = None
+        if self.h_tol is not None:
+            from .fft import HipFFTPlan
+
+            plan = HipFFTPlan(tuple(A.shape), np.complex64, ndim=2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(stats["coarse_steps"]):
@@ -169,9 +217,20 @@ class CorrectedSolver:
             ec.run(Y, 1)
             self.net.correct(Y[None], A[None], out=B[None])
             p = torch.cat((kernels_hip.sumsq(Y[None], 2), kernels_hip.sumsq(B[None], 2))).cpu().numpy()
-            if abs(p[1] - p[0]) / p[0] > self.drift_tol:
+            dp = abs(p[1] - p[0]) / p[0]
+            stats["max_power_drift"] = max(stats["max_power_drift"], float(dp))
+            bad_p = dp > self.drift_tol
+            bad_h = False
+            if not bad_p and plan is not None:
+                # relative to |T| + |U|: H itself may be near zero (focusing)
+                (ty, uy), (tb, ub) = hamiltonian_terms(sim, Y, plan), hamiltonian_terms(sim, B, plan)
+                dh = float(abs((tb - ub) - (ty - uy)) / (abs(ty) + abs(uy)))
+                stats["max_h_drift"] = max(stats["max_h_drift"], dh)
+                bad_h = dh > self.h_tol
+            if bad_p or bad_h:
                 ef.run(A, self.ratio)
                 stats["fallbacks"] += 1
+                stats["fallbacks_power" if bad_p else "fallbacks_h"] += 1
             else:
                 A.copy_(B)
         if stats["fine_steps"]:

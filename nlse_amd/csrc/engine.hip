@@ -312,6 +312,85 @@ __global__ __launch_bounds__(SUMSQ_THREADS) void k_sumsq_final(const double* par
   if (threadIdx.x == 0) out[item] = red[0];
 }
 
+// Per-item FP64 sums of other point terms, same two passes and order.
+// Weighted: (wx[x] + wy[y]) |A|^2 on an (ny, nx) item (the kinetic term of
+// the Hamiltonian from a transformed field, wx = kx^2, wy = ky^2).
+// Energy: G(|A|^2) + vscale Re(V) |A|^2 with G(I) = g Is (I - Is ln(1 + I/Is)),
+// Is = 1/inv_isat (g I^2 / 2 when inv_isat = 0), the antiderivative of the
+// saturable phase rate g I / (1 + I/Is).
+struct TermWeighted {
+  const float2* A;
+  const double* wx;
+  const double* wy;
+  int64_t nx;
+  __device__ double operator()(int64_t item, int64_t i, int64_t isz) const {
+    const float2 v = A[item * isz + i];
+    const int64_t y = i / nx;
+    return (wx[i - y * nx] + wy[y]) * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+};
+
+struct TermEnergy {
+  const float2* A;
+  double g, inv_isat, vscale;
+  const void* V;
+  int vkind;
+  int64_t v_item_stride;
+  __device__ double operator()(int64_t item, int64_t i, int64_t isz) const {
+    const float2 v = A[item * isz + i];
+    const double I = (double)v.x * v.x + (double)v.y * v.y;
+    double G;
+    if (inv_isat == 0.0) {
+      G = 0.5 * g * I * I;
+    } else {
+      const double x = I * inv_isat;
+      // x - log1p(x), by its series where the difference cancels
+      const double d = x < 1e-4 ? x * x * (0.5 - x * (1.0 / 3.0 - 0.25 * x)) : x - log1p(x);
+      G = g * d / (inv_isat * inv_isat);
+    }
+    double pot = 0.0;
+    if (vkind == V_REAL)
+      pot = (double)static_cast<const float*>(V)[item * v_item_stride + i];
+    else if (vkind == V_CPLX)
+      pot = (double)static_cast<const float2*>(V)[item * v_item_stride + i].x;
+    return G + vscale * pot * I;
+  }
+};
+
+template <class Term>
+__global__ __launch_bounds__(SUMSQ_THREADS) void k_sum_part(Term term, int64_t isz, int64_t nchunks, double* part) {
+  const int64_t item = blockIdx.x / nchunks, ch = blockIdx.x % nchunks;
+  const int64_t beg = ch * SUMSQ_CHUNK, end = beg + SUMSQ_CHUNK < isz ? beg + SUMSQ_CHUNK : isz;
+  double acc = 0.0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += SUMSQ_THREADS) acc += term(item, i, isz);
+  __shared__ double red[SUMSQ_THREADS];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = SUMSQ_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+int scratch_doubles(size_t n, double** out);
+
+template <class Term>
+static int sum_launch(const Term& term, int64_t items, int64_t isz, double* out, void* stream, const char* name) {
+  const int64_t nchunks = (isz + SUMSQ_CHUNK - 1) / SUMSQ_CHUNK;
+  double* part = nullptr;
+  int rc = scratch_doubles((size_t)(items * nchunks), &part);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_sum_part<Term>, dim3((unsigned)(items * nchunks)), dim3(SUMSQ_THREADS), 0, s, term, isz,
+                     nchunks, part);
+  rc = launch_status(name);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sumsq_final, dim3((unsigned)items), dim3(SUMSQ_THREADS), 0, s, part, nchunks, out);
+  return launch_status("k_sumsq_final");
+}
+
+
 // scratch for partial sums, grown on demand, one per device
 std::mutex g_scr_mu;
 std::map<int, std::pair<double*, size_t>> g_scratch;
@@ -507,6 +586,22 @@ int nlse_sumsq(const void* A, int64_t items, int64_t isz, double* out, void* str
 int nlse_sumsq_diff(const void* A, const void* B, int64_t items, int64_t isz, double* out, void* stream) {
   if (!A || !B || !out || items < 1 || isz < 1) return fail(NLSE_ERR_ARG, "bad sumsq_diff arguments");
   return sumsq_launch(A, B, items, isz, out, stream);
+}
+
+int nlse_sumsq_weighted(const void* A, int64_t items, int64_t ny, int64_t nx, const double* wx, const double* wy,
+                        double* out, void* stream) {
+  if (!A || !wx || !wy || !out || items < 1 || ny < 1 || nx < 1) return fail(NLSE_ERR_ARG, "bad sumsq_weighted arguments");
+  const TermWeighted t{(const float2*)A, wx, wy, nx};
+  return sum_launch(t, items, ny * nx, out, stream, "k_sum_part(weighted)");
+}
+
+int nlse_nl_energy(const void* A, int64_t items, int64_t isz, double g, double inv_isat, const void* V, int vkind,
+                   int64_t v_item_stride, double vscale, double* out, void* stream) {
+  if (!A || !out || items < 1 || isz < 1 || inv_isat < 0 || (vkind != V_NONE && !V) ||
+      (vkind != V_NONE && vkind != V_REAL && vkind != V_CPLX) || v_item_stride < 0)
+    return fail(NLSE_ERR_ARG, "bad nl_energy arguments");
+  const TermEnergy t{(const float2*)A, g, inv_isat, vscale, V, vkind, v_item_stride};
+  return sum_launch(t, items, isz, out, stream, "k_sum_part(energy)");
 }
 
 int nlse_delta_n(const void* A, int64_t items, int64_t isz, float qc, float qe, nlse_bparam coef,

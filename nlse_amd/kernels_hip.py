@@ -194,6 +194,49 @@ def sumsq_diff(A: torch.Tensor, B: torch.Tensor, tdims: int) -> torch.Tensor:
     return out.reshape(A.shape[:A.dim() - tdims])
 
 
+def sumsq_weighted(A: torch.Tensor, wx, wy) -> torch.Tensor:
+    """Per-item FP64 sum of (wx[x] + wy[y]) |A[..., y, x]|^2."""
+    _check_field(A)
+    ny, nx = A.shape[-2], A.shape[-1]
+    items = A.numel() // (ny * nx)
+    wx = D.to_device(np.asarray(D.to_host(wx), dtype=np.float64).reshape(-1))
+    wy = D.to_device(np.asarray(D.to_host(wy), dtype=np.float64).reshape(-1))
+    if wx.numel() != nx or wy.numel() != ny:
+        raise ValueError("weights must have nx and ny entries")
+    out = torch.empty(items, dtype=torch.float64, device=A.device)
+    _lib.check(_lib.load().nlse_sumsq_weighted(A.data_ptr(), items, ny, nx, wx.data_ptr(), wy.data_ptr(),
+                                               out.data_ptr(), D.stream_ptr()), "sumsq_weighted")
+    return out.reshape(A.shape[:-2])
+
+
+def nl_energy(A: torch.Tensor, g: float, inv_isat: float, V=None, vscale: float = 0.0) -> torch.Tensor:
+    """Per-item FP64 sum over the last two axes of G(|A|^2) + vscale Re(V) |A|^2,
+    G(I) = g Is (I - Is ln(1 + I/Is)), Is = 1/inv_isat (g I^2/2 when 0)."""
+    _check_field(A)
+    isz = A.shape[-1] * A.shape[-2]
+    items = A.numel() // isz
+    kind, stride, vp = 0, 0, None
+    if V is not None:
+        V = D.to_device(V)
+        if V.dtype == torch.float64:
+            V = V.to(torch.float32)
+        elif V.dtype == torch.complex128:
+            V = V.to(torch.complex64)
+        V = V.contiguous()
+        kind = 1 if V.dtype == torch.float32 else 2
+        if V.numel() == isz:
+            stride = 0
+        elif V.numel() == isz * items:
+            stride = isz
+        else:
+            raise ValueError("V must be one plane or one plane per item")
+        vp = V.data_ptr()
+    out = torch.empty(items, dtype=torch.float64, device=A.device)
+    _lib.check(_lib.load().nlse_nl_energy(A.data_ptr(), items, isz, float(g), float(inv_isat), vp, kind, stride,
+                                          float(vscale), out.data_ptr(), D.stream_ptr()), "nl_energy")
+    return out.reshape(A.shape[:-2])
+
+
 # ------------------------------------------------ callbacks and helpers
 _NUM_KINDS = {torch.float32: 0, torch.float64: 1, torch.int32: 2, torch.int64: 3}
 

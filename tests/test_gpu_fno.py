@@ -1,7 +1,8 @@
 """The learned corrector on the GPU: the BF16 matrix-core GEMM against
 numpy, the network's forward pass and every parameter gradient against the
 FP64 oracle (oracle/fno.py), training on pairs from the plain solver with a
-held-out gain over the uncorrected coarse step, and the drift fallback."""
+held-out gain over the uncorrected coarse step, and the drift fallback
+(power and Hamiltonian)."""
 
 import numpy as np
 import pytest
@@ -110,6 +111,8 @@ def _loop_errors(net, seed, nfine=64):
     coarse = coarse_sim.out_field(E0, (nfine // 8 - 0.5) * 8e-5, verbose=False)
     out, stats = CR.CorrectedSolver(sim(), net, drift_tol=1e-3).out_field(E0, z)
     assert stats["coarse_steps"] == nfine // 8
+    print(f"seed {seed}: fallbacks {stats['fallbacks']} (power {stats['fallbacks_power']}, H {stats['fallbacks_h']}); "
+          f"max drift power {stats['max_power_drift']:.2e}, H {stats['max_h_drift']:.2e}")
     e_c = np.linalg.norm(coarse - fine) / np.linalg.norm(fine)
     e_n = np.linalg.norm(out - fine) / np.linalg.norm(fine)
     return e_c, e_n, (sim, E0, z, fine)
@@ -147,4 +150,11 @@ def test_training_gain_and_fallback(gpu):
     assert stats["fallbacks"] == stats["coarse_steps"] == 8
     # eight runs of 8 fine steps vs one run of 64: equal up to the extra
     # x-transform round trips at the run boundaries
+    assert np.linalg.norm(out - fine) / np.linalg.norm(fine) < 1e-5
+    # a grossly corrupted network with the power test off: the Hamiltonian
+    # test alone catches it
+    net.view("q")[:] = 5000.0
+    out, stats = CR.CorrectedSolver(sim(), net, drift_tol=np.inf, h_tol=1e-3).out_field(E0, z)
+    print(f"corrupted, power test off: {stats}")
+    assert stats["fallbacks_h"] == stats["fallbacks"] == 8 and stats["fallbacks_power"] == 0
     assert np.linalg.norm(out - fine) / np.linalg.norm(fine) < 1e-5

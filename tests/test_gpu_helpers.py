@@ -206,3 +206,33 @@ def test_sumsq_diff_matches_fp64(gpu):
     assert np.allclose(got, want, rtol=1e-12, atol=0)
     A = torch.from_numpy(a).cuda()
     assert np.all(KH.sumsq_diff(A, A.clone(), 2).cpu().numpy() == 0)
+
+
+@pytest.mark.parametrize("isat,vkind,amp", [(np.inf, None, 1.0), (1e5, "real", 1.0), (1e5, "complex", 1.0),
+                                            (1e5, "real", 1e-3)])
+def test_hamiltonian_terms_match_oracle(gpu, isat, vkind, amp):
+    """The corrector's Hamiltonian terms (FP32 transform + FP64 sums on the
+    device, nlse_sumsq_weighted / nlse_nl_energy) against the oracle's FP64
+    restatement; amp = 1e-3 puts every point on the small-I series branch."""
+    from nlse_amd import NLSE
+    from nlse_amd import corrector as CR
+    from oracle import energy as OE
+
+    N = 128
+    kw = dict(alpha=20.0, power=1.05, window=4 * 2.23e-3, n2=-1.6e-9, V=None, L=1e-3, NX=N, NY=N, Isat=isat)
+    g, o = NLSE(**kw), oracle.NLSE(**kw)
+    if vkind is not None:
+        rng = np.random.default_rng(7)
+        V = -1e-4 * np.exp(-(o.XX**2 + o.YY**2) / (300e-6) ** 2)
+        V = V.astype(np.float32) if vkind == "real" else (V + 1j * 1e-5 * rng.random(V.shape)).astype(np.complex64)
+        g.V, o.V = V, V
+    w = 2.23e-3
+    rng = np.random.default_rng(8)
+    E = (np.exp(-(o.XX**2 + o.YY**2) / w**2) * (1 + 0.05 * rng.standard_normal(o.XX.shape))).astype(np.complex64)
+    A, _ = g._prepare_output_array(E, True)
+    if amp != 1.0:
+        A = (A * amp).contiguous()
+    T, U = CR.hamiltonian_terms(g, A)
+    To, Uo = OE.hamiltonian_terms(o, A.cpu().numpy())
+    print(f"isat {isat} V {vkind} amp {amp}: T {T:.6e}/{To:.6e} U {U:.6e}/{Uo:.6e}")
+    assert np.isclose(T, To, rtol=2e-6) and np.isclose(U, Uo, rtol=1e-9)

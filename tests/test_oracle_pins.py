@@ -407,3 +407,60 @@ def test_philox_and_box_muller():
     z0, z1 = CB.box_muller(w[:, 0], w[:, 1])
     z = np.concatenate([z0, z1])
     assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.02 and abs(np.mean(z0 * z1)) < 0.01
+
+
+# ------------------------------------------------------------ Hamiltonian
+def test_hamiltonian_plane_wave_closed_form():
+    """A plane wave a exp(i kx0 x) (grid wavenumber), Kerr, no V:
+    T = kx0^2 a^2 area / 2k, U = g a^4 area / 2."""
+    from oracle import energy as OE
+
+    s = oracle.NLSE(0, 1.05, window, -1.6e-9, None, 1e-3, NX=64, NY=32)
+    a, kx0 = 3.0, s.Kx[5]
+    A = (a * np.exp(1j * kx0 * s.XX)).astype(np.complex128)
+    T, U = OE.hamiltonian_terms(s, A)
+    area = s.NX * s.NY * s.delta_X * s.delta_Y
+    g = s.k / 2 * s.n2 * c * epsilon_0
+    assert np.isclose(T, kx0**2 * a**2 * area / (2 * s.k), rtol=1e-12)
+    assert np.isclose(U, g * a**4 * area / 2, rtol=1e-12)
+
+
+def test_hamiltonian_potential_is_quadrature():
+    """G(I) equals the integral of the phase rate g s/(1 + s/Is) from 0 to
+    I, on both sides of the series switch (I/Is = 1e-4)."""
+    from scipy.integrate import quad
+
+    from oracle import energy as OE
+
+    g, Is = -2.5, 7.0
+    for I in (7e-7, 6e-5 * 7, 1e-3 * 7, 0.5 * 7, 30.0):
+        want = quad(lambda u: g * u / (1 + u / Is), 0, I, epsabs=0, epsrel=1e-13)[0]
+        assert np.isclose(OE.G(I, g, Is), want, rtol=1e-11, atol=0), I
+    assert np.isclose(OE.G(2.0, g, np.inf), g * 2.0, rtol=1e-15)
+
+
+def test_hamiltonian_conserved_by_lossless_propagation():
+    """alpha = 0, saturable, real V: the oracle's own split steps keep H
+    within 1e-4 of |T| + |U| while the kinetic part and the V part each
+    move by more than 1e-3 of it (a wrong sign or factor in any term
+    breaks the balance)."""
+    from oracle import energy as OE
+
+    N = 128
+    s = oracle.NLSE(alpha=0, power=1.05, window=window, n2=-1.6e-9, V=None, L=1e-2, NX=N, NY=N, Isat=1e5,
+                    mode="c128")
+    s.V = (-3e-4 * np.exp(-(s.XX**2 + s.YY**2) / (400e-6) ** 2)).astype(np.float32)
+    s.delta_z = 2e-5
+    rng = np.random.default_rng(0)
+    E0 = (np.exp(-(s.XX**2 + s.YY**2) / waist**2) * (1 + 0.05 * rng.standard_normal(s.XX.shape))).astype(
+        np.complex64)
+    A0, _ = s._prepare_output_array(E0, True)
+    out = s.out_field(E0, 5e-3)
+    T0, U0 = OE.hamiltonian_terms(s, A0)
+    T1, U1 = OE.hamiltonian_terms(s, out)
+    scale = abs(T0) + abs(U0)
+    cell = s.delta_X * s.delta_Y
+    uv0 = cell * (s.k / 2 * s.V * np.abs(A0) ** 2).sum()
+    uv1 = cell * (s.k / 2 * s.V * np.abs(out) ** 2).sum()
+    assert abs((T1 - U1) - (T0 - U0)) / scale < 1e-4
+    assert abs(T1 - T0) / scale > 1e-3 and abs(uv1 - uv0) / scale > 1e-3
