@@ -355,6 +355,14 @@ int nlse_batch_run_pass(nlse_batch* batch, void* base_dev, int pass, int64_t cou
 int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, int64_t sak, int64_t sa,
                   const float* B, int64_t sbk, int64_t sbn, int64_t sb, float* C, int64_t scm, int64_t scn,
                   int64_t sc, float alpha, float beta, const float* bias, float* act, void* stream);
+/* Pointwise (1x1) channel mixing over hw pixels of channel-major planes:
+ * Y[b][o][p] = sum_c X[b][c][p] W[c][o] + beta Y[b][o][p] [+ bias[o]],
+ * act != NULL also stores gelu(Y); operands rounded to BF16, FP32
+ * accumulation (the products of nlse_fno_gemm with M = hw, N = cout,
+ * K = cin).  (cin, cout) in {(4,32), (32,32), (32,2), (4,16), (16,16),
+ * (16,2), (4,8), (8,8), (8,2)}; other shapes return NLSE_ERR_ARG. */
+int nlse_fno_pointwise(const float* X, const float* W, const float* bias, float* Y, float* act, int B, int cin,
+                       int cout, int64_t hw, float beta, void* stream);
 /* Per-mode complex channel mixing of a spectral layer (32 x 32 modes):
  * P1 [B*Cin][64][64] -> Q [B*Cout][64][64] with R [1024][Cout][Cin][2]. */
 int nlse_fno_mix(const float* P1, float* Q, const float* R, int B, int Cin, int Cout, void* stream);

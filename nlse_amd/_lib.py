@@ -109,6 +109,8 @@ SIGNATURES = {
     "nlse_fno_gemm": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_i64, c_i64, c_i64, c_void_p, c_i64, c_i64,
                               c_i64, c_void_p, c_i64, c_i64, c_i64, C.c_float, C.c_float, c_void_p, c_void_p,
                               c_void_p]),
+    "nlse_fno_pointwise": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_i64,
+                                   C.c_float, c_void_p]),
     "nlse_fno_mix": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "nlse_fno_mix_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "nlse_fno_gelu_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_void_p]),

@@ -58,6 +58,12 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // C = alpha * A B + beta * C [+ bias], 64x64 tile per block of 4 waves,
 // each wave a 32x32 quarter as 2x2 MFMA 16x16 tiles; K in steps of 32.
+// SWAP computes the transposed tile (B^T A^T, the same products): the 16
+// lanes of a result row then run along m instead of n, so a C whose m
+// index is the contiguous one (channel-major pointwise products over
+// pixels) is read and written in 64-byte runs rather than one float per
+// channel plane.
+template <bool SWAP>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   __shared__ unsigned short As[GBM][GBK + GPAD];
   __shared__ unsigned short Bs[GBN][GBK + GPAD];  // stored transposed: Bs[n][k]
@@ -130,7 +136,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) {
+          if constexpr (SWAP)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[j], a[i], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
@@ -143,8 +154,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q;
-        const int n = n0 + wn + j * 16 + (lane & 15);
+        // result lane l holds row 4*(l/16)+q, column l%16 of its 16x16 tile
+        // (of C, or of C^T with SWAP)
+        const int m = m0 + wm + i * 16 + (SWAP ? (lane & 15) : 4 * (lane >> 4) + q);
+        const int n = n0 + wn + j * 16 + (SWAP ? 4 * (lane >> 4) + q : (lane & 15));
         if (m >= g.M || n >= g.N) continue;
         const int64_t off = (int64_t)m * g.scm + (int64_t)n * g.scn;
         float v = g.alpha * acc[i][j][q];
@@ -153,6 +166,60 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         C[off] = v;
         if (Act) Act[off] = gelu_f(v);
       }
+}
+
+// ------------------------------------------------------ pointwise layer
+// Y[b][o][p] = sum_c X[b][c][p] W[c][o] + beta Y[b][o][p] + bias[o] over
+// hw pixels (channel-major planes), one pixel per thread: the 1x1 channel
+// mixing of the lift, projection and the pointwise path of every layer.
+// Operands are rounded to BF16 like the matrix-core products (weights once
+// into LDS, inputs on load) and accumulated in FP32.  A layer moves
+// (CIN + COUT, + 2 COUT with beta and the GELU output) x 4 bytes per pixel
+// for CIN x COUT products, so it is bound by HBM and the VALU together, not
+// by matrix-core throughput: as a 64x64-tile product it left half of each
+// tile empty and spent most of its time staging (1024^2, 32 -> 32 channels
+// with beta and GELU: 324 us as a tile product, 111 us here against a
+// 64 us HBM floor; four pixels per thread measured slower, 127 us).
+__device__ __forceinline__ float bf16_round(float x) { return __uint_as_float((unsigned int)to_bf16(x) << 16); }
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_fno_pointwise(const float* __restrict__ X, const float* __restrict__ W,
+                                                       const float* __restrict__ bias, float* __restrict__ Y,
+                                                       float* __restrict__ act, int64_t hw, float beta) {
+  __shared__ float w[CIN * COUT];
+  for (int i = threadIdx.x; i < CIN * COUT; i += 256) w[i] = bf16_round(W[i]);
+  __syncthreads();
+  const int64_t b = blockIdx.y;
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= hw) return;
+  const float* x = X + b * CIN * hw + p;
+  float acc[COUT];
+#pragma unroll
+  for (int o = 0; o < COUT; ++o) acc[o] = 0.f;
+#pragma unroll 4
+  for (int c = 0; c < CIN; ++c) {
+    const float v = bf16_round(x[(int64_t)c * hw]);
+#pragma unroll
+    for (int o = 0; o < COUT; ++o) acc[o] = fmaf(v, w[c * COUT + o], acc[o]);
+  }
+  float* y = Y + b * COUT * hw + p;
+  float* a = act ? act + b * COUT * hw + p : nullptr;
+#pragma unroll
+  for (int o = 0; o < COUT; ++o) {
+    float v = acc[o];
+    if (beta != 0.f) v += beta * y[(int64_t)o * hw];
+    if (bias) v += bias[o];
+    y[(int64_t)o * hw] = v;
+    if (a) a[(int64_t)o * hw] = gelu_f(v);
+  }
+}
+
+template <int CIN, int COUT>
+static int pointwise_launch(const float* X, const float* W, const float* bias, float* Y, float* act, int B,
+                            int64_t hw, float beta, hipStream_t s) {
+  hipLaunchKernelGGL((k_fno_pointwise<CIN, COUT>), dim3((unsigned)((hw + 255) / 256), (unsigned)B), dim3(256), 0, s,
+                     X, W, bias, Y, act, hw, beta);
+  return launch_status("k_fno_pointwise");
 }
 
 // ------------------------------------------------------- spectral mixing
@@ -383,8 +450,28 @@ int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, i
   GemmArgs g{M, N, K, batch, A, sam, sak, sa, B, sbk, sbn, sb, C, scm, scn, sc, alpha, beta, bias, act};
   const dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
   if (grid.y > 65535) return fail(NLSE_ERR_ARG, "gemm M too large (%d)", M);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, (hipStream_t)stream, g);
+  if (scm == 1 && scn != 1)
+    hipLaunchKernelGGL(k_gemm_bf16<true>, grid, dim3(256), 0, (hipStream_t)stream, g);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(256), 0, (hipStream_t)stream, g);
   return launch_status("k_gemm_bf16");
+}
+
+int nlse_fno_pointwise(const float* X, const float* W, const float* bias, float* Y, float* act, int B, int cin,
+                       int cout, int64_t hw, float beta, void* stream) {
+  if (!X || !W || !Y || B < 1 || B > 65535 || hw < 1 || (hw + 255) / 256 > 0x7fffffff)
+    return fail(NLSE_ERR_ARG, "bad pointwise arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (cin == 4 && cout == 32) return pointwise_launch<4, 32>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 32 && cout == 32) return pointwise_launch<32, 32>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 32 && cout == 2) return pointwise_launch<32, 2>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 4 && cout == 16) return pointwise_launch<4, 16>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 16 && cout == 16) return pointwise_launch<16, 16>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 16 && cout == 2) return pointwise_launch<16, 2>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 4 && cout == 8) return pointwise_launch<4, 8>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 8 && cout == 8) return pointwise_launch<8, 8>(X, W, bias, Y, act, B, hw, beta, s);
+  if (cin == 8 && cout == 2) return pointwise_launch<8, 2>(X, W, bias, Y, act, B, hw, beta, s);
+  return fail(NLSE_ERR_ARG, "pointwise layer %d -> %d channels not instantiated", cin, cout);
 }
 
 int nlse_fno_mix(const float* P1, float* Q, const float* R, int B, int Cin, int Cout, void* stream) {

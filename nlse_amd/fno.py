@@ -145,9 +145,16 @@ class FNO:
                                           B.data_ptr(), sbk, sbn, sb, C.data_ptr(), scm, scn, sc, 1.0,
                                           float(beta), _f(bias), _f(act), D.stream_ptr()), "fno_gemm")
 
+    # (cin, cout) pairs with a dedicated pointwise kernel (nlse_fno_pointwise)
+    _PW = {(4, 32), (32, 32), (32, 2), (4, 16), (16, 16), (16, 2), (4, 8), (8, 8), (8, 2)}
+
     def _pointwise(self, X, Wt, bias, Y, B, cin, cout, beta=0.0, act=None):
         """Y[b][o][p] = sum_c X[b][c][p] Wt[c][o] (+ bias[o]) for all b."""
         hw = self.H * self.W
+        if (cin, cout) in self._PW:
+            _lib.check(self.lib.nlse_fno_pointwise(X.data_ptr(), Wt.data_ptr(), _f(bias), Y.data_ptr(), _f(act), B,
+                                                   cin, cout, hw, float(beta), D.stream_ptr()), "fno_pointwise")
+            return
         self._gemm(hw, cout, cin, B, X, 1, hw, cin * hw, Wt, cout, 1, 0, Y, 1, hw, cout * hw, beta, bias, act)
 
     def _weight_grad(self, X, dY, B, cin, cout, out):

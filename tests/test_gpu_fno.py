@@ -20,11 +20,16 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("M,N,K,batch,ta,tb", [(100, 70, 300, 1, False, False), (64, 64, 64, 3, True, False),
-                                               (33, 130, 1000, 2, False, True), (4096, 32, 32, 1, True, True)])
-def test_gemm_bf16_matches_numpy(gpu, M, N, K, batch, ta, tb):
+@pytest.mark.parametrize("M,N,K,batch,ta,tb,tc", [(100, 70, 300, 1, False, False, False),
+                                                  (64, 64, 64, 3, True, False, False),
+                                                  (33, 130, 1000, 2, False, True, False),
+                                                  (4096, 32, 32, 1, True, True, False),
+                                                  (4096, 32, 32, 2, True, False, True),
+                                                  (1000, 70, 45, 1, False, False, True)])
+def test_gemm_bf16_matches_numpy(gpu, M, N, K, batch, ta, tb, tc):
     """C = A B + 0.5 C + bias with BF16-rounded operands and FP32
-    accumulation, for row- and column-major operands, batched, with the
+    accumulation, for row- and column-major operands and results (a
+    column-major C takes the transposed-tile kernel), batched, with the
     GELU side output."""
     from nlse_amd import _lib
 
@@ -36,18 +41,22 @@ def test_gemm_bf16_matches_numpy(gpu, M, N, K, batch, ta, tb):
     bias = rng.standard_normal(N).astype(np.float32)
     Ad = torch.from_numpy(np.ascontiguousarray(A.transpose(0, 2, 1)) if ta else A).cuda()
     Bd = torch.from_numpy(np.ascontiguousarray(B.transpose(0, 2, 1)) if tb else B).cuda()
-    Cd = torch.from_numpy(C0).cuda()
+    Cd = torch.from_numpy(np.ascontiguousarray(C0.transpose(0, 2, 1)) if tc else C0).cuda()
     act = torch.empty_like(Cd)
     bd = torch.from_numpy(bias).cuda()
     sam, sak = (1, M) if ta else (K, 1)
     sbk, sbn = (1, K) if tb else (N, 1)
+    scm, scn = (1, M) if tc else (N, 1)
     rc = lib.nlse_fno_gemm(M, N, K, batch, Ad.data_ptr(), sam, sak, M * K, Bd.data_ptr(), sbk, sbn, K * N,
-                           Cd.data_ptr(), N, 1, M * N, 1.0, 0.5, bd.data_ptr(), act.data_ptr(), 0)
+                           Cd.data_ptr(), scm, scn, M * N, 1.0, 0.5, bd.data_ptr(), act.data_ptr(), 0)
     assert rc == 0, lib.nlse_last_error()
     ref = np.einsum("bmk,bkn->bmn", _bf16(A), _bf16(B)) + 0.5 * C0 + bias
     got = Cd.cpu().numpy()
+    ga = act.cpu().numpy()
+    if tc:
+        got, ga = got.transpose(0, 2, 1), ga.transpose(0, 2, 1)
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
-    assert np.abs(act.cpu().numpy() - F.gelu(got.astype(np.float64))).max() < 1e-5
+    assert np.abs(ga - F.gelu(got.astype(np.float64))).max() < 1e-5
 
 
 def _net_and_params(H, W, C, L, seed=3):
@@ -158,3 +167,39 @@ def test_training_gain_and_fallback(gpu):
     print(f"corrupted, power test off: {stats}")
     assert stats["fallbacks_h"] == stats["fallbacks"] == 8 and stats["fallbacks_power"] == 0
     assert np.linalg.norm(out - fine) / np.linalg.norm(fine) < 1e-5
+
+
+@pytest.mark.parametrize("cin,cout,beta,with_act", [(4, 32, 0.0, False), (32, 32, 1.0, True), (32, 2, 0.0, False),
+                                                    (16, 16, 1.0, True)])
+def test_pointwise_matches_gemm_and_numpy(gpu, cin, cout, beta, with_act):
+    """nlse_fno_pointwise (the 1x1 layers) against numpy with BF16-rounded
+    operands and against nlse_fno_gemm on the same product."""
+    from nlse_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(cin * 100 + cout)
+    B, hw = 2, 3000
+    X = rng.standard_normal((B, cin, hw)).astype(np.float32)
+    W = rng.standard_normal((cin, cout)).astype(np.float32)
+    Y0 = rng.standard_normal((B, cout, hw)).astype(np.float32)
+    bias = rng.standard_normal(cout).astype(np.float32)
+    Xd, Wd, bd = (torch.from_numpy(a).cuda() for a in (X, W, bias))
+    outs = []
+    for kind in ("pointwise", "gemm"):
+        Yd = torch.from_numpy(Y0.copy()).cuda()
+        act = torch.empty_like(Yd) if with_act else None
+        ap = act.data_ptr() if with_act else None
+        if kind == "pointwise":
+            rc = lib.nlse_fno_pointwise(Xd.data_ptr(), Wd.data_ptr(), bd.data_ptr(), Yd.data_ptr(), ap, B, cin, cout,
+                                        hw, beta, 0)
+        else:
+            rc = lib.nlse_fno_gemm(hw, cout, cin, B, Xd.data_ptr(), 1, hw, cin * hw, Wd.data_ptr(), cout, 1, 0,
+                                   Yd.data_ptr(), 1, hw, cout * hw, 1.0, beta, bd.data_ptr(), ap, 0)
+        assert rc == 0, lib.nlse_last_error()
+        outs.append((Yd.cpu().numpy(), act.cpu().numpy() if with_act else None))
+    ref = np.einsum("bcp,co->bop", _bf16(X), _bf16(W)) + beta * Y0 + bias[None, :, None]
+    for y, a in outs:
+        assert np.abs(y - ref).max() / np.abs(ref).max() < 1e-5
+        if with_act:
+            assert np.abs(a - F.gelu(y.astype(np.float64))).max() < 1e-5
+    assert np.abs(outs[0][0] - outs[1][0]).max() / np.abs(ref).max() < 1e-6
