@@ -12,6 +12,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "../../include/nlse_hip.h"
 #include "common.hpp"
@@ -39,6 +41,10 @@ struct GemmArgs {
   float alpha, beta;
   const float* bias;  // [N] or null
   float* act;         // if set: act(m, n) = gelu(C(m, n)) (same strides)
+  // split-K: blockIdx.z = b * ksplit + s takes k in [s kchunk, (s+1) kchunk)
+  // and stores alpha * partial to part[blockIdx.z][m][n] (no epilogue)
+  int ksplit, kchunk;
+  float* part;
 };
 
 constexpr int GBM = 64, GBN = 64, GBK = 32;
@@ -70,7 +76,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const int64_t bz = blockIdx.z;
+  const int64_t bz = blockIdx.z / g.ksplit;
+  const int split = (int)(blockIdx.z % g.ksplit);
+  const int kbeg = split * g.kchunk;
+  const int kend = g.ksplit > 1 ? min(g.K, kbeg + g.kchunk) : g.K;
   const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
   const float* A = g.A + bz * g.sa;
   const float* B = g.B + bz * g.sb;
@@ -84,7 +93,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
 
   const bool a_kfast = g.sak == 1;
   const bool b_nfast = g.sbn == 1;
-  for (int k0 = 0; k0 < g.K; k0 += GBK) {
+  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
     // stage the A (64 x 32) and B (32 x 64) tiles, 8 elements per thread,
     // lanes along the contiguous index of each operand
 #pragma unroll
@@ -99,7 +108,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         mm = e % GBM;
       }
       const int m = m0 + mm, k = k0 + kk;
-      const float v = (m < g.M && k < g.K) ? A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.f;
+      const float v = (m < g.M && k < kend) ? A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.f;
       As[mm][kk] = to_bf16(v);
     }
 #pragma unroll
@@ -114,7 +123,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         kk = e % GBK;
       }
       const int n = n0 + nn, k = k0 + kk;
-      const float v = (n < g.N && k < g.K) ? B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.f;
+      const float v = (n < g.N && k < kend) ? B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.f;
       Bs[nn][kk] = to_bf16(v);
     }
     __syncthreads();
@@ -159,6 +168,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         const int m = m0 + wm + i * 16 + (SWAP ? (lane & 15) : 4 * (lane >> 4) + q);
         const int n = n0 + wn + j * 16 + (SWAP ? 4 * (lane >> 4) + q : (lane & 15));
         if (m >= g.M || n >= g.N) continue;
+        if (g.ksplit > 1) {
+          g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = g.alpha * acc[i][j][q];
+          continue;
+        }
         const int64_t off = (int64_t)m * g.scm + (int64_t)n * g.scn;
         float v = g.alpha * acc[i][j][q];
         if (g.beta != 0.f) v += g.beta * C[off];
@@ -166,6 +179,24 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         C[off] = v;
         if (Act) Act[off] = gelu_f(v);
       }
+}
+
+// Split-K epilogue: C(m, n) = sum_s part[b*S + s][m][n] + beta C(m, n)
+// [+ bias[n]], act = gelu(C), in a fixed order (deterministic).
+__global__ void k_gemm_splitk_reduce(GemmArgs g) {
+  const int64_t mn = (int64_t)g.M * g.N;
+  const int64_t total = (int64_t)g.batch * mn;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / mn, r = i - b * mn;
+    const int m = (int)(r / g.N), n = (int)(r - (int64_t)m * g.N);
+    float v = 0.f;
+    for (int sp = 0; sp < g.ksplit; ++sp) v += g.part[(b * g.ksplit + sp) * mn + r];
+    const int64_t off = b * g.sc + (int64_t)m * g.scm + (int64_t)n * g.scn;
+    if (g.beta != 0.f) v += g.beta * g.C[off];
+    if (g.bias) v += g.bias[n];
+    g.C[off] = v;
+    if (g.act) g.act[off] = gelu_f(v);
+  }
 }
 
 // ------------------------------------------------------ pointwise layer
@@ -434,6 +465,31 @@ __global__ void k_fno_mse(const float* __restrict__ out, const float* __restrict
 
 using namespace nlse;
 
+// Split-K partial sums: one device buffer per GPU, grown on demand and
+// kept until the process ends.  Calls on one stream are ordered; calls
+// that use it from concurrent streams of one GPU would race.
+static std::mutex g_ws_mu;
+static std::map<int, std::pair<float*, size_t>> g_ws;
+
+static int ws_floats(size_t n, float** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(NLSE_ERR_HIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto& e = g_ws[dev];
+  if (e.second < n) {
+    // the previous buffer may still be read by queued work on this stream
+    if (e.first && hipDeviceSynchronize() != hipSuccess) return fail(NLSE_ERR_HIP, "synchronize failed");
+    if (e.first) (void)hipFree(e.first);
+    e.first = nullptr;
+    e.second = 0;
+    if (hipMalloc((void**)&e.first, n * sizeof(float)) != hipSuccess)
+      return fail(NLSE_ERR_HIP, "hipMalloc of the split-K workspace failed");
+    e.second = n;
+  }
+  *out = e.first;
+  return NLSE_OK;
+}
+
 static unsigned grid1d(int64_t n, int threads = 256) {
   int64_t b = (n + threads - 1) / threads;
   return (unsigned)(b < 1 ? 1 : (b > 65535 * 16 ? 65535 * 16 : b));
@@ -447,14 +503,33 @@ int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, i
   if (M < 0 || N < 0 || K < 0 || batch < 1 || batch > 65535 || !A || !B || !C)
     return fail(NLSE_ERR_ARG, "bad gemm arguments (%d x %d x %d, batch %d)", M, N, K, batch);
   if (M == 0 || N == 0) return NLSE_OK;
-  GemmArgs g{M, N, K, batch, A, sam, sak, sa, B, sbk, sbn, sb, C, scm, scn, sc, alpha, beta, bias, act};
-  const dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)batch);
+  GemmArgs g{M, N, K, batch, A, sam, sak, sa, B, sbk, sbn, sb, C, scm, scn, sc, alpha, beta, bias, act, 1, K,
+             nullptr};
+  const int64_t tiles = (int64_t)((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM) * batch;
+  // Split K when the tiles alone would leave the chip mostly idle with a
+  // long K loop each (the truncated DFTs: K = 1024 over 32..512 tiles):
+  // slices of >= 64 (two staging rounds), enough to reach ~1024 blocks
+  // (a 4096-block target measured slower on the 512-tile x transform).
+  int S = 1;
+  while (S < 16 && tiles * S < 1024 && K / (2 * S) >= 64) S *= 2;
+  if (S > 1 && (int64_t)batch * S <= 65535) {
+    const int chunk = ((K + S - 1) / S + GBK - 1) / GBK * GBK;
+    int rc = ws_floats((size_t)batch * S * M * N, &g.part);
+    if (rc) return rc;
+    g.ksplit = S;
+    g.kchunk = chunk;
+  }
+  const dim3 grid((unsigned)((N + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM), (unsigned)(batch * g.ksplit));
   if (grid.y > 65535) return fail(NLSE_ERR_ARG, "gemm M too large (%d)", M);
   if (scm == 1 && scn != 1)
     hipLaunchKernelGGL(k_gemm_bf16<true>, grid, dim3(256), 0, (hipStream_t)stream, g);
   else
     hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(256), 0, (hipStream_t)stream, g);
-  return launch_status("k_gemm_bf16");
+  int rc = launch_status("k_gemm_bf16");
+  if (rc || g.ksplit == 1) return rc;
+  hipLaunchKernelGGL(k_gemm_splitk_reduce, dim3(grid1d((int64_t)batch * M * N)), dim3(256), 0, (hipStream_t)stream,
+                     g);
+  return launch_status("k_gemm_splitk_reduce");
 }
 
 int nlse_fno_pointwise(const float* X, const float* W, const float* bias, float* Y, float* act, int B, int cin,
