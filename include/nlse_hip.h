@@ -351,7 +351,11 @@ int nlse_batch_run_pass(nlse_batch* batch, void* base_dev, int pass, int64_t cou
 /* C(m,n) = alpha sum_k A(m,k) B(k,n) + beta C(m,n) [+ bias[n]], batched:
  * A(m,k) = A[b*sa + m*sam + k*sak], B(k,n) = B[b*sb + k*sbk + n*sbn],
  * C(m,n) = C[b*sc + m*scm + n*scn].  Operands FP32 in memory, rounded to
- * BF16 on load.  act != NULL also stores gelu(C) at the same offsets. */
+ * BF16 on load.  act != NULL also stores gelu(C) at the same offsets.
+ * Products with few output tiles and a long K split K into slices whose
+ * partial sums go to a library workspace (one per GPU, grown on demand,
+ * kept until the process ends) and are summed in a fixed order: calls on
+ * one stream are ordered, calls from concurrent streams of one GPU race. */
 int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, int64_t sak, int64_t sa,
                   const float* B, int64_t sbk, int64_t sbn, int64_t sb, float* C, int64_t scm, int64_t scn,
                   int64_t sc, float alpha, float beta, const float* bias, float* act, void* stream);
