@@ -95,7 +95,13 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   const bool b_nfast = g.sbn == 1;
   for (int k0 = kbeg; k0 < kend; k0 += GBK) {
     // stage the A (64 x 32) and B (32 x 64) tiles, 8 elements per thread,
-    // lanes along the contiguous index of each operand
+    // lanes along the contiguous index of each operand.  All 16 loads are
+    // issued before any is used, from clamped addresses, and out-of-range
+    // elements are zeroed after by a multiply (a select lets the compiler
+    // sink a load into a branch, which then waits for it at the join).
+    float va[8], vb[8];
+    float ma[8], mb[8];
+    int la[8], lb[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int e = r * 256 + tid;
@@ -108,8 +114,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         mm = e % GBM;
       }
       const int m = m0 + mm, k = k0 + kk;
-      const float v = (m < g.M && k < kend) ? A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.f;
-      As[mm][kk] = to_bf16(v);
+      const int mc = m < g.M ? m : g.M - 1, kc = k < kend ? k : kend - 1;
+      va[r] = A[(int64_t)mc * g.sam + (int64_t)kc * g.sak];
+      ma[r] = (float)(m < g.M && k < kend);
+      la[r] = mm * (GBK + GPAD) + kk;
     }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -123,8 +131,15 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         kk = e % GBK;
       }
       const int n = n0 + nn, k = k0 + kk;
-      const float v = (n < g.N && k < kend) ? B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.f;
-      Bs[nn][kk] = to_bf16(v);
+      const int nc = n < g.N ? n : g.N - 1, kc = k < kend ? k : kend - 1;
+      vb[r] = B[(int64_t)kc * g.sbk + (int64_t)nc * g.sbn];
+      mb[r] = (float)(n < g.N && k < kend);
+      lb[r] = nn * (GBK + GPAD) + kk;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      (&As[0][0])[la[r]] = to_bf16(va[r] * ma[r]);
+      (&Bs[0][0])[lb[r]] = to_bf16(vb[r] * mb[r]);
     }
     __syncthreads();
     // v_mfma_f32_16x16x16_bf16 operands: lane l holds A[l%16][4*(l/16)+i]
