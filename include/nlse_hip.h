@@ -361,7 +361,8 @@ int nlse_fno_gemm(int M, int N, int K, int batch, const float* A, int64_t sam, i
                   int64_t sc, float alpha, float beta, const float* bias, float* act, void* stream);
 /* Pointwise (1x1) channel mixing over hw pixels of channel-major planes:
  * Y[b][o][p] = sum_c X[b][c][p] W[c][o] + beta Y[b][o][p] [+ bias[o]],
- * act != NULL also stores gelu(Y); operands rounded to BF16, FP32
+ * act != NULL also stores gelu(Y) (act == Y: only gelu(Y), in place);
+ * operands rounded to BF16, FP32
  * accumulation (the products of nlse_fno_gemm with M = hw, N = cout,
  * K = cin).  (cin, cout) in {(4,32), (32,32), (32,2), (4,16), (16,16),
  * (16,2), (4,8), (8,8), (8,2)}; other shapes return NLSE_ERR_ARG. */

@@ -250,13 +250,19 @@ __global__ __launch_bounds__(256) void k_fno_pointwise(const float* __restrict__
   }
   float* y = Y + b * COUT * hw + p;
   float* a = act ? act + b * COUT * hw + p : nullptr;
+  // act == Y: only gelu(Y) is stored (inference keeps no pre-activation)
+  const bool inplace = act == Y;
 #pragma unroll
   for (int o = 0; o < COUT; ++o) {
     float v = acc[o];
     if (beta != 0.f) v += beta * y[(int64_t)o * hw];
     if (bias) v += bias[o];
-    y[(int64_t)o * hw] = v;
-    if (a) a[(int64_t)o * hw] = gelu_f(v);
+    if (inplace) {
+      y[(int64_t)o * hw] = gelu_f(v);
+    } else {
+      y[(int64_t)o * hw] = v;
+      if (a) a[(int64_t)o * hw] = gelu_f(v);
+    }
   }
 }
 

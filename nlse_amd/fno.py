@@ -200,9 +200,13 @@ class FNO:
                                              D.stream_ptr()), "fno_mix")
             U = self._buf("U", (B * C * H, 64))
             self._gemm(H, 64, 64, B * C, self.Fy, 1, H, 0, Q, 64, 1, 4096, U, 64, 1, H * 64)
-            z = self._buf(f"z_{l}" if keep else "z", (B, C, H, W))
+            # inference keeps no pre-activation: the layer's GELU is applied
+            # in place, alternating two buffers (h is the other one)
+            inplace = not keep and not last
+            zname = f"z_{l}" if keep else (("hA" if l % 2 == 0 else "hB") if inplace else "z")
+            z = self._buf(zname, (B, C, H, W))
             self._gemm(B * C * H, W, 64, 1, U, 64, 1, 0, self.Fx, 1, 64, 0, z, W, 1, 0)
-            hn = self._buf(f"h{l + 1}" if keep else ("hA" if l % 2 == 0 else "hB"), (B, C, H, W))
+            hn = z if inplace else self._buf(f"h{l + 1}" if keep else ("hA" if l % 2 == 0 else "hB"), (B, C, H, W))
             self._pointwise(h, self.view(f"W{l}"), self.view(f"b{l}"), z, B, C, C, beta=1.0,
                             act=None if last else hn)
             if last:

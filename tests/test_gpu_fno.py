@@ -96,6 +96,9 @@ def test_fno_forward_and_gradients_match_oracle(gpu):
         e = np.linalg.norm(g[o:o + n] - gref[o:o + n]) / (np.linalg.norm(gref[o:o + n]) + 1e-30)
         print(f"grad {name}: {e:.2e}")
         assert e < 3e-2, name
+    # inference (no saved activations, GELU in place) gives the same output
+    out_inf = net.forward(torch.from_numpy(x).cuda(), keep=False).cpu().numpy()
+    assert np.array_equal(out_inf, out)
 
 
 def _loop_errors(net, seed, nfine=64):
