@@ -142,6 +142,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end out_field timings")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--width", type=int, default=32, help="corrector: channels of the network")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one process "
@@ -490,7 +491,7 @@ def bench_corrector(args) -> int:
 
     t0 = time.perf_counter()
     pairs = CR.make_pairs(N=128, trajectories=80, coarse_steps=20, seed=1)
-    net = FNO(128, 128, width=32, layers=4, seed=0)
+    net = FNO(128, 128, width=args.width, layers=4, seed=0)
     CR.train(net, pairs, steps=800, batch=8, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -529,7 +530,7 @@ def bench_corrector(args) -> int:
     E0 = CR._speckle_field(sf, rng)
     A, ef = CR._prepared(sf, E0)
     _, ec = CR._prepared(sc, E0)
-    big = FNO(N, N, width=32, layers=4, seed=0)
+    big = FNO(N, N, width=args.width, layers=4, seed=0)
     big.res_scale = net.res_scale
     ef.run(A, 16)
     t_fine = _timed(ef.run, A, 8 * args.steps) / (8 * args.steps)
@@ -563,7 +564,7 @@ def bench_corrector(args) -> int:
         "vs_baseline": None,
         "dtype": "c64 solver, bf16 MFMA network (fp32 accumulation)",
         "data": "synthetic",
-        "config": {"workload": "FNO corrector (4 spectral layers, 32 channels, 32x32 modes) of coarse "
+        "config": {"workload": f"FNO corrector (4 spectral layers, {args.width} channels, 32x32 modes) of coarse "
                                "steps dz_c = 8e-5 vs fine dz = 1e-5; trained at 128^2 on 1600 pairs from the "
                                "plain solver (README-example material x U(0.5,1.5), speckle inputs); throughput at "
                                "1024^2",
