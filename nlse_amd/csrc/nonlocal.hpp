@@ -193,7 +193,9 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
   if (!a.direct) {
     const int64_t zrow = FULL ? L.item * a.ny + L.y[0] : gs;
 #pragma unroll
-    for (int m = 0; m < P; ++m) z[0][m] = active ? a.Z[zrow * N + t + m * T] : make_float2(0.f, 0.f);
+    // unconditional: lines past the end read line 0 (gs) and store nothing;
+    // a select per element would put each load behind a branch
+    for (int m = 0; m < P; ++m) z[0][m] = a.Z[zrow * N + t + m * T];
   }
   float2 v[NC][P];
   load_lines<NC, P, T>(a.A, L.off, t, active, v);
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
       float2 z2[1][P];
 #pragma unroll
       for (int m = 0; m < P; ++m)
-        z2[0][m] = active ? a.Z[(L.item * a.ny + L.y[1]) * N + t + m * T] : make_float2(0.f, 0.f);
+        z2[0][m] = a.Z[(L.item * a.ny + L.y[1]) * N + t + m * T];
       fft_line<N, P, 1, true>(z2, t, a.tw, lline, LINE);
 #pragma unroll
       for (int m = 0; m < P; ++m) z[0][m].y = z2[0][m].x;
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
 #pragma unroll
     for (int m = 0; m < P; ++m) {
       u[0][m] = v[0][m];
-      u[1][m] = active ? a.A[L.off[1] + t + m * T] : make_float2(0.f, 0.f);
+      u[1][m] = a.A[L.off[1] + t + m * T];
       I[m] = z[0][m].y;
     }
     load_row_v(L.y[1]);
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(LPB*(N / P)) void k_nlc_nl(NlcArgs a) {
     float2 w[1][P];
 #pragma unroll
     for (int m = 0; m < P; ++m) {
-      w[0][m] = active ? a.A[L.off[1] + t + m * T] : make_float2(0.f, 0.f);
+      w[0][m] = a.A[L.off[1] + t + m * T];
       I[m] = z[0][m].y;
     }
     load_row_v(L.y[1]);
